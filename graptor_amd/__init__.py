@@ -1,0 +1,362 @@
+"""graptor_amd -- Python bindings of the gfx950 edgemap/vertexmap engine.
+
+The engine is the C-ABI library ``graptor_amd/libgpr.so`` (declared in
+``include/gpr.h``); this module loads it with ctypes.  There is no CPU
+fallback: importing works without a GPU (so that the CPU test suite can
+check the exported symbols), but every compute call requires the library
+and a gfx950 device and raises otherwise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpr.so")
+
+GPR_OK, GPR_E_INVAL, GPR_E_FORMAT, GPR_E_OOM, GPR_E_HIP, GPR_E_IO, GPR_E_NODEV = range(7)
+U32, F32, F64, U64, U16 = range(5)
+FRONTIER_EMPTY, FRONTIER_ALL, FRONTIER_LIST, FRONTIER_BITMAP = range(4)
+RELAX_MIN_PLUS1, RELAX_MIN, RELAX_PR_SUM = range(3)
+DST_NONE, DST_GT, DST_NE = range(3)
+RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE = range(3)
+DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
+CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T = range(5)
+VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER = range(4)
+CONTRIB_NAMES = {"fp32": CONTRIB_F32, "bf16": CONTRIB_BF16, "fp16": CONTRIB_F16, "cfp16": CONTRIB_CFP16,
+                 "bf16t": CONTRIB_BF16T}
+
+
+class GprError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("gpr error %d: %s" % (code, msg))
+        self.code = code
+
+
+class EmapDesc(C.Structure):
+    _fields_ = [("relax", C.c_int), ("prop", C.c_void_p), ("prev", C.c_void_p), ("src_strong", C.c_int),
+                ("dst_filter", C.c_int), ("dst_arg", C.c_uint32), ("record", C.c_int), ("direction", C.c_int),
+                ("out_kind", C.c_int), ("threshold_pct", C.c_float), ("contrib", C.c_void_p),
+                ("contrib_fmt", C.c_int), ("damping", C.c_float)]
+
+
+class IterStats(C.Structure):
+    _fields_ = [("nactv_in", C.c_uint32), ("nacte_in", C.c_uint64), ("nactv_out", C.c_uint32),
+                ("nacte_out", C.c_uint64), ("direction", C.c_int), ("out_kind", C.c_int),
+                ("edges_scanned", C.c_uint64), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class RunOpts(C.Structure):
+    _fields_ = [("direction", C.c_int), ("out_kind", C.c_int), ("threshold_pct", C.c_float),
+                ("max_iter", C.c_int), ("tol", C.c_double), ("trace", C.c_int)]
+
+
+class RunInfo(C.Structure):
+    _fields_ = [("iters", C.c_int), ("push_iters", C.c_int), ("edges", C.c_uint64), ("ms", C.c_double),
+                ("delta", C.c_double), ("sum", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PrStage(C.Structure):
+    _fields_ = [("contrib", C.c_int), ("tol", C.c_double)]
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+# name -> (restype, argtypes); every entry point declared in include/gpr.h
+SIGNATURES = {
+    "gpr_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "gpr_version": (C.c_char_p, []),
+    "gpr_init": (C.c_int, [C.c_int]),
+    "gpr_graph_create": (C.c_int, [_P, _P, C.c_uint32, C.c_uint64, C.c_int, _PP]),
+    "gpr_graph_load_b2": (C.c_int, [C.c_char_p, C.c_int, _PP]),
+    "gpr_graph_save_b2": (C.c_int, [_P, C.c_char_p]),
+    "gpr_graph_generate_rmat": (C.c_int, [C.c_uint32, C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                          C.c_uint64, _PP]),
+    "gpr_graph_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    "gpr_graph_download": (C.c_int, [_P, _P, _P]),
+    "gpr_graph_highest_degree_vertex": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
+    "gpr_graph_destroy": (C.c_int, [_P]),
+    "gpr_prop_create": (C.c_int, [_P, C.c_int, _PP]),
+    "gpr_prop_upload": (C.c_int, [_P, _P]),
+    "gpr_prop_download": (C.c_int, [_P, _P]),
+    "gpr_prop_destroy": (C.c_int, [_P]),
+    "gpr_frontier_create": (C.c_int, [_P, _PP]),
+    "gpr_frontier_set_single": (C.c_int, [_P, C.c_uint32]),
+    "gpr_frontier_set_all": (C.c_int, [_P]),
+    "gpr_frontier_set_list": (C.c_int, [_P, _P, C.c_uint32]),
+    "gpr_frontier_set_neighbours": (C.c_int, [_P, C.c_uint32]),
+    "gpr_frontier_stats": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
+    "gpr_frontier_convert": (C.c_int, [_P, C.c_int]),
+    "gpr_frontier_download": (C.c_int, [_P, _P, C.POINTER(C.c_uint32)]),
+    "gpr_frontier_destroy": (C.c_int, [_P]),
+    "gpr_edgemap": (C.c_int, [_P, C.POINTER(EmapDesc), _P, _P, C.POINTER(IterStats)]),
+    "gpr_vertex_map": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_uint64]),
+    "gpr_vertex_filter": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_int, _P]),
+    "gpr_pr_contrib": (C.c_int, [_P, _P, _P, C.c_int, C.c_float]),
+    "gpr_pr_finish": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_float, C.POINTER(C.c_double),
+                                C.POINTER(C.c_double)]),
+    "gpr_pr_scalars": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_float)]),
+    "gpr_bfslvl": (C.c_int, [_P, C.c_uint32, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_ccv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_prv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_pr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_trace_get": (C.c_int, [_P, C.POINTER(IterStats), C.c_int, C.POINTER(C.c_int)]),
+    "gpr_profile_enable": (C.c_int, [_P, C.c_int]),
+    "gpr_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    "gpr_profile_reset": (C.c_int, [_P]),
+}
+
+_lib = None
+_inited = False
+
+
+def load_library(path=LIB_PATH):
+    """Load libgpr.so (no device initialisation).  Raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise GprError(GPR_E_NODEV, "%s not built: run __graft_entry__.build()" % path)
+        L = C.CDLL(path)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def lib():
+    """The library with the gfx950 device initialised."""
+    global _inited
+    L = load_library()
+    if not _inited:
+        _check(L.gpr_init(0), L)
+        _inited = True
+    return L
+
+
+def _check(rc, L=None):
+    if rc != GPR_OK:
+        L = L or _lib
+        buf = C.create_string_buffer(2048)
+        L.gpr_last_error(buf, len(buf))
+        raise GprError(rc, buf.value.decode(errors="replace"))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def opts(direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, max_iter=100, tol=1e-7, trace=False):
+    return RunOpts(direction, out_kind, threshold_pct, max_iter, tol, 1 if trace else 0)
+
+
+class Graph:
+    """A device-resident graph (CSR + CSC + pull partitions)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle) if not isinstance(handle, C.c_void_p) else handle
+        n = C.c_uint32()
+        m = C.c_uint64()
+        s = C.c_int()
+        _check(lib().gpr_graph_info(self.h, C.byref(n), C.byref(m), C.byref(s)))
+        self.n, self.m, self.symmetric = n.value, m.value, bool(s.value)
+
+    @classmethod
+    def from_csr(cls, index, nbrs, symmetric=True):
+        index = np.ascontiguousarray(index, np.uint64)
+        nbrs = np.ascontiguousarray(nbrs, np.uint32)
+        h = C.c_void_p()
+        _check(lib().gpr_graph_create(_ptr(index), _ptr(nbrs) if len(nbrs) else None, len(index) - 1, len(nbrs),
+                                      1 if symmetric else 0, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def load_b2(cls, path, symmetric=True):
+        h = C.c_void_p()
+        _check(lib().gpr_graph_load_b2(path.encode(), 1 if symmetric else 0, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0x47524150):
+        h = C.c_void_p()
+        _check(lib().gpr_graph_generate_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)))
+        return cls(h)
+
+    def save_b2(self, path):
+        _check(lib().gpr_graph_save_b2(self.h, path.encode()))
+
+    def download(self):
+        index = np.empty(self.n + 1, np.uint64)
+        nbrs = np.empty(max(1, self.m), np.uint32)
+        _check(lib().gpr_graph_download(self.h, _ptr(index), _ptr(nbrs)))
+        return index, nbrs[: self.m]
+
+    def highest_degree_vertex(self):
+        v = C.c_uint32()
+        _check(lib().gpr_graph_highest_degree_vertex(self.h, C.byref(v)))
+        return v.value
+
+    def destroy(self):
+        if self.h:
+            _check(lib().gpr_graph_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
+
+    # ---- applications -------------------------------------------------------
+    def bfslvl(self, start, **kw):
+        out = np.empty(self.n, np.uint32)
+        o = opts(**kw)
+        info = RunInfo()
+        _check(lib().gpr_bfslvl(self.h, start, C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
+    def ccv(self, **kw):
+        out = np.empty(self.n, np.uint32)
+        o = opts(**kw)
+        info = RunInfo()
+        _check(lib().gpr_ccv(self.h, C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
+    def prv(self, max_iter=20, tol=0.0, **kw):
+        out = np.empty(self.n, np.float32)
+        o = opts(max_iter=max_iter, tol=tol, **kw)
+        info = RunInfo()
+        _check(lib().gpr_prv(self.h, C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
+    def pr_vary(self, stages, max_iter=100, **kw):
+        """stages: list of (format name or CONTRIB_*, tol)."""
+        arr = (PrStage * len(stages))(*[PrStage(CONTRIB_NAMES.get(f, f), t) for f, t in stages])
+        out = np.empty(self.n, np.float32)
+        o = opts(max_iter=max_iter, **kw)
+        info = RunInfo()
+        _check(lib().gpr_pr_vary(self.h, arr, len(stages), C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
+    def trace(self):
+        k = C.c_int()
+        _check(lib().gpr_trace_get(self.h, None, 0, C.byref(k)))
+        buf = (IterStats * max(1, k.value))()
+        _check(lib().gpr_trace_get(self.h, buf, k.value, C.byref(k)))
+        return [buf[i].as_dict() for i in range(k.value)]
+
+    def profile(self, on=True):
+        _check(lib().gpr_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self):
+        _check(lib().gpr_profile_reset(self.h))
+
+    def profile_get(self, name):
+        ms = C.c_double()
+        cnt = C.c_uint64()
+        _check(lib().gpr_profile_get(self.h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    # ---- engine-level access --------------------------------------------------
+    def prop(self, dtype=U32):
+        return Prop(self, dtype)
+
+    def frontier(self):
+        return Frontier(self)
+
+    def edgemap(self, relax, prop, frontier_in=None, frontier_out=None, dst_filter=DST_NONE, dst_arg=0,
+                record=RECORD_REDUCTION, direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, prev=None,
+                contrib=None, contrib_fmt=CONTRIB_F32, damping=0.85):
+        d = EmapDesc(relax, prop.h, prev.h if prev else None, 0, dst_filter, dst_arg, record, direction, out_kind,
+                     threshold_pct, contrib.h if contrib else None, contrib_fmt, damping)
+        st = IterStats()
+        _check(lib().gpr_edgemap(self.h, C.byref(d), frontier_in.h if frontier_in else None,
+                                 frontier_out.h if frontier_out else None, C.byref(st)))
+        return st.as_dict()
+
+
+_NP = {U32: np.uint32, F32: np.float32, F64: np.float64, U64: np.uint64, U16: np.uint16}
+
+
+class Prop:
+    def __init__(self, g, dtype):
+        self.g, self.dtype = g, dtype
+        self.h = C.c_void_p()
+        _check(lib().gpr_prop_create(g.h, dtype, C.byref(self.h)))
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, _NP[self.dtype])
+        assert a.size == self.g.n
+        _check(lib().gpr_prop_upload(self.h, _ptr(a)))
+
+    def download(self):
+        a = np.empty(self.g.n, _NP[self.dtype])
+        _check(lib().gpr_prop_download(self.h, _ptr(a)))
+        return a
+
+    def vertex_map(self, op, value_bits=0, other=None, frontier=None):
+        _check(lib().gpr_vertex_map(self.g.h, op, self.h, other.h if other else None,
+                                    frontier.h if frontier else None, value_bits))
+
+    def destroy(self):
+        if self.h:
+            _check(lib().gpr_prop_destroy(self.h))
+            self.h = None
+
+
+class Frontier:
+    def __init__(self, g):
+        self.g = g
+        self.h = C.c_void_p()
+        _check(lib().gpr_frontier_create(g.h, C.byref(self.h)))
+
+    def set_single(self, v):
+        _check(lib().gpr_frontier_set_single(self.h, v))
+        return self
+
+    def set_all(self):
+        _check(lib().gpr_frontier_set_all(self.h))
+        return self
+
+    def set_list(self, vs):
+        a = np.ascontiguousarray(vs, np.uint32)
+        _check(lib().gpr_frontier_set_list(self.h, _ptr(a) if len(a) else None, len(a)))
+        return self
+
+    def set_neighbours(self, v):
+        _check(lib().gpr_frontier_set_neighbours(self.h, v))
+        return self
+
+    def stats(self):
+        v = C.c_uint32()
+        e = C.c_uint64()
+        k = C.c_int()
+        _check(lib().gpr_frontier_stats(self.h, C.byref(v), C.byref(e), C.byref(k)))
+        return v.value, e.value, k.value
+
+    def convert(self, kind):
+        _check(lib().gpr_frontier_convert(self.h, kind))
+
+    def download(self):
+        nactv = self.stats()[0]
+        out = np.empty(max(1, nactv), np.uint32)
+        k = C.c_uint32()
+        _check(lib().gpr_frontier_download(self.h, _ptr(out), C.byref(k)))
+        return out[: k.value]
+
+    def destroy(self):
+        if self.h:
+            _check(lib().gpr_frontier_destroy(self.h))
+            self.h = None
+
+
+def version():
+    return load_library().gpr_version().decode()

@@ -1,0 +1,320 @@
+// apps.cpp -- the callers of the hot path, written against gpr::api the way
+// the reference benchmarks are written against Graptor's api:: templates.
+//
+//   BFSLVLv   bench/BFSLVLv.C:110-254  (run loop :154-251)
+//   CCv       bench/CCv.C:221-588      (run :221-471, emap_step :520-588)
+//   PRv       bench/PRv.C:115-290      (iteration :201-280)
+//   PR_vary   bench/PR_vary.C:247-757  (run_prec :247-447, staged run :652-757)
+//
+// Each benchmark runs the reference's DEFERRED_UPDATE=0 configuration of
+// the record operator (api::record(output, api::reduction, api::strong),
+// BFSLVLv.C:173 and CCv.C:543-549): a vertex joins the next frontier when
+// its relax changed it.  For BFS levels and min-label CC this activates the
+// same vertices as the changed-vs-previous method and needs no prev copy.
+#include "../../include/gpr/api.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+struct gpr_graph;
+namespace gpr {
+std::vector<gpr_iter_stats> &trace_of(gpr_graph *g);
+void trace_clear(gpr_graph *g);
+double stream_time_ms(gpr_graph *g, bool start);
+}
+
+using namespace gpr;
+namespace A = gpr::api;
+
+namespace {
+
+gpr_run_opts defaults(const gpr_run_opts *o) {
+    gpr_run_opts d = {};
+    d.direction = GPR_DIR_AUTO;
+    d.threshold_pct = 5.f;
+    d.max_iter = 100;
+    d.tol = 1e-7;
+    if (o) {
+        d = *o;
+        if (d.threshold_pct <= 0.f) d.threshold_pct = 5.f;
+        if (d.max_iter <= 0) d.max_iter = 100;
+    }
+    return d;
+}
+
+void record_step(gpr_graph *g, const gpr_run_opts &o, const gpr_iter_stats &st, gpr_run_info *info) {
+    if (o.trace) trace_of(g).push_back(st);
+    info->iters++;
+    if (st.direction == GPR_DIR_PUSH) info->push_iters++;
+}
+
+template <class F> int guarded(F &&f) {
+    try {
+        f();
+        return GPR_OK;
+    } catch (const gpr::error &e) {
+        return e.code;
+    } catch (const std::exception &e) {
+        check(GPR_E_INVAL, e.what());
+        return GPR_E_INVAL;
+    }
+}
+
+// ---- BFSLVLv -----------------------------------------------------------------
+struct BFSLVLv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<uint32_t> a_level;
+    int iter = 0;
+    BFSLVLv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), a_level(GA, "current level") {}
+    ~BFSLVLv() { a_level.del(); }
+
+    void run(uint32_t start, gpr_run_info *info) {
+        const uint32_t n = GA.numVertices();
+        // BFSLVLv.C:117-129: ilvl = n+1 everywhere, 0 at the start vertex
+        A::make_lazy_executor(GA).vertex_map_fill(a_level, n + 1).materialize();
+        A::frontier F = A::frontier::create(GA, start);
+        A::make_lazy_executor(GA).vertex_map_fill(F, a_level, 0).materialize();
+        info->edges += F.nActiveEdges();
+        iter = 0;
+        while (!F.isEmpty()) {
+            A::frontier output;
+            auto lazy = A::edgemap(GA,
+                                   A::record(output, A::reduction, A::strong),
+                                   A::filter(A::weak, A::src, F),
+                                   A::filter(A::weak, A::dst, A::gt(a_level, (uint32_t)iter + 1)),
+                                   A::relax(A::min_plus1(a_level)),
+                                   A::config(o.direction, o.threshold_pct, o.out_kind));
+            lazy.materialize();
+            record_step(GA.get(), o, lazy.stats(), info);
+            info->edges += output.nActiveEdges();
+            F.del();
+            F = output;
+            ++iter;
+        }
+        F.del();
+    }
+};
+
+// ---- CCv -----------------------------------------------------------------------
+struct CCv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<uint32_t> m_IDs;
+    int iter = 0;
+    CCv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), m_IDs(GA, "IDs") {}
+    ~CCv() { m_IDs.del(); }
+
+    // CCv.C:520-588
+    A::frontier emap_step(A::frontier &F, float threshold, gpr_run_info *info) {
+        A::frontier output;
+        int dir = o.direction;
+        float pct = threshold;
+        auto lazy = A::edgemap(GA,
+                               A::config(dir, pct, o.out_kind),
+                               A::record(output, A::reduction, A::strong),
+                               A::filter(A::weak, A::src, F),
+                               A::filter(A::weak, A::dst, A::ne(m_IDs, 0)),
+                               A::relax(A::min_label(m_IDs)));
+        lazy.materialize();
+        record_step(GA.get(), o, lazy.stats(), info);
+        return output;
+    }
+
+    A::frontier find_zeros(A::frontier &F) {
+        A::frontier z;
+        A::make_lazy_executor(GA).vertex_filter(F, z, m_IDs, 0, false).materialize();
+        return z;
+    }
+    A::frontier find_nonzeros(A::frontier &F) {
+        A::frontier z;
+        A::make_lazy_executor(GA).vertex_filter(F, z, m_IDs, 0, true).materialize();
+        return z;
+    }
+
+    void run(gpr_run_info *info) {
+        const uint32_t n = GA.numVertices();
+        // CCv.C:246-248 IDs[v] = v
+        A::make_lazy_executor(GA).vertex_map_iota(m_IDs).materialize();
+        iter = 0;
+        uint32_t initial_zeros = 1;
+        // PRESET_ZERO, CCv.C:262-291: IDs[N(0)] = 0
+        if (n > 0) {
+            A::frontier G0 = A::frontier::empty(GA);
+            check(gpr_frontier_set_neighbours(G0.get(), 0), "neighbours of 0");
+            A::make_lazy_executor(GA).vertex_map_fill(G0, m_IDs, 0).materialize();
+            initial_zeros += G0.nActiveVertices();
+            G0.del();
+            ++iter;
+        }
+        bool switched = false, have_checked = false;
+        A::frontier zeros;
+        A::frontier F = A::frontier::all_true(GA);
+        while (!F.isEmpty()) {
+            A::frontier output = emap_step(F, o.threshold_pct, info);
+            // PUSH_ZERO, CCv.C:373-388: after one step, if few vertices carry
+            // label 0, push the zero label first and the rest afterwards
+            if (!have_checked) {
+                zeros = find_zeros(output);
+                if ((uint64_t(zeros.nActiveVertices()) + initial_zeros) * 10 < n) {
+                    F.del();
+                    F = output;
+                    ++iter;
+                    switched = true;
+                    break;
+                }
+                zeros.del();
+                have_checked = true;
+            }
+            F.del();
+            F = output;
+            ++iter;
+        }
+        F.del();
+        if (switched) {
+            // CCv.C:398-469, frac_threshold(200): always sparse
+            while (!zeros.isEmpty()) {
+                A::frontier Z = zeros;
+                A::frontier output = emap_step(Z, 200.f, info);
+                ++iter;
+                Z.del();
+                zeros = find_zeros(output);
+                output.del();
+            }
+            zeros.del();
+            A::frontier all = A::frontier::all_true(GA);
+            A::frontier nonzeros = find_nonzeros(all);
+            all.del();
+            while (!nonzeros.isEmpty()) {
+                A::frontier Z = nonzeros;
+                A::frontier output = emap_step(Z, 200.f, info);
+                ++iter;
+                Z.del();
+                nonzeros = find_nonzeros(output);
+                output.del();
+            }
+            nonzeros.del();
+        }
+        info->edges = GA.numEdges();
+    }
+};
+
+// ---- PRv / PR_vary --------------------------------------------------------------
+struct PRv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<float> x_mem, y_mem;
+    PRv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), x_mem(GA, "x"), y_mem(GA, "y") {}
+    ~PRv() {
+        x_mem.del();
+        y_mem.del();
+    }
+
+    // run_prec: one precision stage, continuing from x (PR_vary.C:247-447)
+    template <class CT>
+    void run_prec(int fmt, double tol, int &iter, double &sdelta, gpr_run_info *info) {
+        A::vertexprop<CT> contrib(GA, "contrib");
+        const float dampen = 0.85f;
+        check(gpr_pr_contrib(GA.get(), x_mem.get(), contrib.get(), fmt, dampen), "contributions");
+        sdelta = 2.0;
+        const bool need_delta = tol > 0.0;
+        while (iter < o.max_iter && sdelta > tol) {
+            auto lazy = A::edgemap(GA, A::relax(A::pr_sum(y_mem, contrib, fmt, dampen)));
+            lazy.materialize();
+            // the vertex_scan / vertex_map tail of PRv.C:214-269, fused
+            check(gpr_pr_finish(GA.get(), x_mem.get(), y_mem.get(), contrib.get(), fmt, dampen,
+                                need_delta ? &sdelta : nullptr, nullptr),
+                  "pr tail");
+            gpr_iter_stats st = {};
+            st.nactv_in = GA.numVertices();
+            st.nacte_in = GA.numEdges();
+            st.direction = GPR_DIR_PULL;
+            st.edges_scanned = GA.numEdges();
+            record_step(GA.get(), o, st, info);
+            info->edges += GA.numEdges();
+            ++iter;
+        }
+        contrib.del();
+    }
+
+    void run_stage(int fmt, double tol, int &iter, double &sdelta, gpr_run_info *info) {
+        if (fmt == GPR_CONTRIB_F32)
+            run_prec<float>(fmt, tol, iter, sdelta, info);
+        else
+            run_prec<uint16_t>(fmt, tol, iter, sdelta, info);
+    }
+
+    void run(const gpr_pr_stage *stages, int nstages, gpr_run_info *info) {
+        const uint32_t n = GA.numVertices();
+        // PRv.C:163-170: x = 1/n
+        A::make_lazy_executor(GA).vertex_map_fill(x_mem, 1.0f / (float)n).materialize();
+        int iter = 0;
+        double sdelta = 2.0;
+        for (int s = 0; s < nstages; ++s) run_stage(stages[s].contrib, stages[s].tol, iter, sdelta, info);
+        double sx = 0.0;
+        check(gpr_pr_scalars(GA.get(), nullptr, &sdelta, &sx, nullptr), "pr scalars");
+        info->delta = sdelta;
+        info->sum = sx;
+    }
+};
+
+} // namespace
+
+extern "C" int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *opts, uint32_t *levels,
+                          gpr_run_info *info) {
+    if (!g) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    uint32_t n = 0;
+    gpr_graph_info(g, &n, nullptr, nullptr);
+    if (start >= n) return guarded([&] { check(GPR_E_INVAL, "bfslvl: start vertex out of range"); });
+    return guarded([&] {
+        BFSLVLv app(g, o);
+        gpr_run_info local = {};
+        if (o.trace) trace_clear(g);
+        stream_time_ms(g, true);
+        app.run(start, &local);
+        local.ms = stream_time_ms(g, false);
+        if (levels) app.a_level.download(levels);
+        if (info) *info = local;
+    });
+}
+
+extern "C" int gpr_ccv(gpr_graph *g, const gpr_run_opts *opts, uint32_t *labels, gpr_run_info *info) {
+    if (!g) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    return guarded([&] {
+        CCv app(g, o);
+        gpr_run_info local = {};
+        if (o.trace) trace_clear(g);
+        stream_time_ms(g, true);
+        app.run(&local);
+        local.ms = stream_time_ms(g, false);
+        if (labels) app.m_IDs.download(labels);
+        if (info) *info = local;
+    });
+}
+
+extern "C" int gpr_pr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages, const gpr_run_opts *opts, float *x,
+                           gpr_run_info *info) {
+    if (!g || !stages || nstages <= 0) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    return guarded([&] {
+        PRv app(g, o);
+        gpr_run_info local = {};
+        if (o.trace) trace_clear(g);
+        stream_time_ms(g, true);
+        app.run(stages, nstages, &local);
+        local.ms = stream_time_ms(g, false);
+        if (x) app.x_mem.download(x);
+        if (info) *info = local;
+    });
+}
+
+extern "C" int gpr_prv(gpr_graph *g, const gpr_run_opts *opts, float *x, gpr_run_info *info) {
+    gpr_run_opts o = defaults(opts);
+    gpr_pr_stage st = {GPR_CONTRIB_F32, o.tol};
+    return gpr_pr_vary(g, &st, 1, &o, x, info);
+}

@@ -1,0 +1,69 @@
+// device.hpp -- wave64 helpers for gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gpr {
+
+__device__ __forceinline__ unsigned lane() { return __lane_id(); }
+
+// Number of set bits of `mask` in lanes below this one.
+__device__ __forceinline__ unsigned mbcnt(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+template <class T> __device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <class T> __device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T t = __shfl_xor(v, o, 64);
+        v = t < v ? t : v;
+    }
+    return v;
+}
+template <class T> __device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T t = __shfl_xor(v, o, 64);
+        v = t > v ? t : v;
+    }
+    return v;
+}
+// Inclusive prefix sum across the wave.
+template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
+    unsigned l = lane();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T t = __shfl_up(v, o, 64);
+        if (l >= (unsigned)o) v += t;
+    }
+    return v;
+}
+
+// Block-wide sum of a per-thread value; result valid in every thread.
+// `red` must hold blockDim.x/64 elements.
+template <class T> __device__ __forceinline__ T block_sum(T v, T *red) {
+    v = wave_sum(v);
+    unsigned w = threadIdx.x / 64, nw = blockDim.x / 64;
+    __syncthreads();
+    if (lane() == 0) red[w] = v;
+    __syncthreads();
+    T s = 0;
+    for (unsigned i = 0; i < nw; ++i) s += red[i];
+    return s;
+}
+
+__device__ __forceinline__ bool dst_ok(uint32_t v, int kind, uint32_t arg) {
+    return kind == GPR_DST_GT ? v > arg : kind == GPR_DST_NE ? v != arg : true;
+}
+
+__device__ __forceinline__ bool bit_test(const uint64_t *bits, uint32_t v) {
+    return (bits[v >> 6] >> (v & 63)) & 1ull;
+}
+
+} // namespace gpr

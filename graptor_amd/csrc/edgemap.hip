@@ -1,0 +1,580 @@
+// edgemap.hip -- push and pull traversal kernels and the edgemap dispatcher.
+//
+// Reference semantics:
+//   dispatcher      api::edgemap, include/graptor/api.h:2143-2415, with the
+//                   thresholds frac_threshold / default_threshold :130-160
+//   sparse push     csr_sparse_with_f, dsl/edgemap.h:1397-1740, edge-balanced
+//                   blocks of EMAP_BLOCK_SIZE edges (:29-31, :1434-1436) and
+//                   edge_partition::process_push, dsl/emap/edgechunk.h:347-410
+//   dense pull      GraphCSx_csc scalar pull, dsl/emap/GraphCSx_csc.h:50-140
+//                   (destination filter re-checked per edge, :110-111)
+//   relax           redop_min / atomic_min CAS returning "changed",
+//                   dsl/ast/redop.h:628-805, simd/vector_ref.h:205-225
+//
+// GPU design:
+//   push: the frontier's out-degrees are prefix-summed (two small kernels),
+//         then each workgroup takes a tile of PUSH_THREADS*PUSH_ITEMS
+//         consecutive frontier edges (load-balanced search: the tile's
+//         frontier slice is staged in LDS and each lane binary-searches it),
+//         so hubs and leaves cost the same per edge and neighbour reads are
+//         coalesced.  Relax is a u32 atomicMin; first activation is decided
+//         by atomicOr on a bitmap word (output bitmap or a dedup bitmap).
+//   pull: one wave per 64 consecutive destinations, lane per destination,
+//         so a bitmap output word is one ballot and one store.  Lanes with
+//         at most 32 in-edges loop alone with early exit; the wave walks the
+//         in-edges of every heavier destination together (64 edges per
+//         step, wave-min, early exit once the destination filter fails).
+#include "internal.hpp"
+#include "device.hpp"
+
+namespace gpr {
+
+// ---- degree prefix of a frontier list -------------------------------------
+constexpr int SCAN_THREADS = SCAN_BLOCK / 4;
+
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan_local(const uint32_t *__restrict__ list, uint32_t k, const uint32_t *__restrict__ deg,
+             uint64_t *__restrict__ pos, uint64_t *__restrict__ btot) {
+    __shared__ uint64_t wsum[SCAN_THREADS / 64];
+    uint32_t base = blockIdx.x * SCAN_BLOCK + threadIdx.x * 4;
+    uint64_t d[4];
+    uint64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t i = base + j;
+        d[j] = i < k ? deg[list[i]] : 0;
+        s += d[j];
+    }
+    uint64_t inc = wave_incl_scan(s);
+    unsigned w = threadIdx.x / 64;
+    if (lane() == 63) wsum[w] = inc;
+    __syncthreads();
+    uint64_t off = 0;
+    for (unsigned j = 0; j < w; ++j) off += wsum[j];
+    uint64_t run = off + inc - s;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint32_t i = base + j;
+        if (i < k) pos[i] = run;
+        run += d[j];
+    }
+    if (threadIdx.x == SCAN_THREADS - 1) btot[blockIdx.x] = off + inc;
+}
+
+// Exclusive scan of the block totals in place; total written to a[count].
+__global__ void __launch_bounds__(1024) k_scan_top(uint64_t *a, uint32_t count) {
+    __shared__ uint64_t carry;
+    __shared__ uint64_t wsum[16];
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t b = 0; b < count; b += blockDim.x) {
+        uint32_t i = b + threadIdx.x;
+        uint64_t v = i < count ? a[i] : 0;
+        uint64_t inc = wave_incl_scan(v);
+        unsigned w = threadIdx.x / 64;
+        if (lane() == 63) wsum[w] = inc;
+        __syncthreads();
+        uint64_t off = carry;
+        for (unsigned j = 0; j < w; ++j) off += wsum[j];
+        if (i < count) a[i] = off + inc - v;
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = off + inc;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) a[count] = carry;
+}
+
+__device__ __forceinline__ uint64_t frontier_pos(const uint64_t *pos, const uint64_t *btot, uint32_t i) {
+    return pos[i] + btot[i / SCAN_BLOCK];
+}
+
+// First index i in [0,k) with pos(i) > e (k if none).
+__device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *btot, uint32_t k, uint64_t e) {
+    uint32_t lo = 0, hi = k;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (frontier_pos(pos, btot, mid) <= e) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) {
+    return RELAX == GPR_RELAX_MIN_PLUS1 ? ps + 1u : ps;
+}
+
+// ---- push ------------------------------------------------------------------
+constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ITEMS;
+constexpr int PUSH_LDS = PUSH_TILE + 1;
+
+template <int RELAX>
+__global__ void __launch_bounds__(PUSH_THREADS)
+k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restrict__ pos,
+       const uint64_t *__restrict__ btot, uint64_t E, const uint64_t *__restrict__ index,
+       const uint32_t *__restrict__ nbrs, const uint32_t *__restrict__ deg, uint32_t *prop, int dkind,
+       uint32_t darg, uint64_t *bits, uint32_t *out_list, DevCounters *ctr) {
+    __shared__ uint64_t s_pos[PUSH_LDS];
+    __shared__ uint32_t s_v[PUSH_LDS];
+    __shared__ uint32_t q[PUSH_TILE];
+    __shared__ uint32_t s_i0, s_cnt, qn, qbase;
+    __shared__ uint64_t red[PUSH_THREADS / 64];
+    __shared__ uint32_t red32[PUSH_THREADS / 64];
+    const uint64_t ntiles = (E + PUSH_TILE - 1) / PUSH_TILE;
+    uint32_t my_v = 0;
+    uint64_t my_e = 0;
+    for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const uint64_t e0 = t * PUSH_TILE;
+        const uint64_t e1 = min(E, e0 + PUSH_TILE);
+        if (threadIdx.x == 0) {
+            uint32_t i0 = ub_pos(pos, btot, k, e0) - 1;
+            uint32_t i1 = ub_pos(pos, btot, k, e1 - 1) - 1;
+            s_i0 = i0;
+            s_cnt = i1 - i0 + 1;
+            qn = 0;
+        }
+        __syncthreads();
+        const uint32_t i0 = s_i0, cnt = s_cnt;
+        const bool in_lds = cnt <= (uint32_t)PUSH_LDS;
+        if (in_lds)
+            for (uint32_t j = threadIdx.x; j < cnt; j += PUSH_THREADS) {
+                s_pos[j] = frontier_pos(pos, btot, i0 + j);
+                s_v[j] = flist[i0 + j];
+            }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < PUSH_ITEMS; ++it) {
+            const uint64_t e = e0 + (uint64_t)it * PUSH_THREADS + threadIdx.x;
+            bool act = false;
+            uint32_t d = 0;
+            if (e < e1) {
+                uint64_t pj;
+                uint32_t s;
+                if (in_lds) {
+                    uint32_t lo = 0, hi = cnt;
+                    while (lo < hi) {
+                        uint32_t mid = (lo + hi) >> 1;
+                        if (s_pos[mid] <= e) lo = mid + 1; else hi = mid;
+                    }
+                    pj = s_pos[lo - 1];
+                    s = s_v[lo - 1];
+                } else {
+                    uint32_t j = ub_pos(pos, btot, k, e) - 1;
+                    pj = frontier_pos(pos, btot, j);
+                    s = flist[j];
+                }
+                d = nbrs[index[s] + (e - pj)];
+                const uint32_t pd = prop[d];
+                if (dst_ok(pd, dkind, darg)) {
+                    const uint32_t nv = relax_val<RELAX>(prop[s]);
+                    if (nv < pd) {
+                        const uint32_t old = atomicMin(&prop[d], nv);
+                        if (old > nv) {
+                            const uint64_t bit = 1ull << (d & 63);
+                            const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
+                            act = !(ow & bit);
+                        }
+                    }
+                }
+            }
+            if (act) {
+                ++my_v;
+                my_e += deg[d];
+                if (out_list) q[atomicAdd(&qn, 1u)] = d;
+            }
+        }
+        __syncthreads();
+        if (out_list) {
+            const uint32_t c = qn;
+            if (threadIdx.x == 0 && c) qbase = atomicAdd(&ctr->listpos, c);
+            __syncthreads();
+            for (uint32_t j = threadIdx.x; j < c; j += PUSH_THREADS) out_list[qbase + j] = q[j];
+        }
+        __syncthreads();
+    }
+    const uint32_t tv = block_sum(my_v, red32);
+    const uint64_t te = block_sum(my_e, red);
+    if (threadIdx.x == 0) {
+        if (tv) atomicAdd(&ctr->nactv, tv);
+        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+    }
+}
+
+// ---- pull ------------------------------------------------------------------
+constexpr int PULL_THREADS = 256;
+constexpr uint32_t PULL_SERIAL_MAX = 32;
+
+template <int RELAX>
+__global__ void __launch_bounds__(PULL_THREADS)
+k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
+       const uint32_t *__restrict__ deg, uint32_t *prop, const uint32_t *__restrict__ prev, int dkind,
+       uint32_t darg, int record, const uint64_t *__restrict__ in_bits, uint64_t *out_bits,
+       uint32_t *out_list, DevCounters *ctr) {
+    __shared__ uint64_t red[PULL_THREADS / 64];
+    __shared__ uint32_t red32[PULL_THREADS / 64];
+    const uint32_t ngroups = (n + 63) / 64;
+    const uint32_t wid = (blockIdx.x * PULL_THREADS + threadIdx.x) / 64;
+    const uint32_t nwaves = gridDim.x * (PULL_THREADS / 64);
+    const unsigned ln = lane();
+    uint32_t my_v = 0;
+    uint64_t my_e = 0, my_scan = 0;
+    for (uint32_t grp = wid; grp < ngroups; grp += nwaves) {
+        const uint32_t d = grp * 64 + ln;
+        const bool valid = d < n;
+        const uint32_t orig = valid ? prop[d] : 0u;
+        uint32_t cur = orig;
+        const bool act = valid && dst_ok(cur, dkind, darg);
+        uint64_t beg = 0, end = 0;
+        if (act) {
+            beg = cindex[d];
+            end = cindex[d + 1];
+        }
+        const bool heavy = act && (end - beg) > PULL_SERIAL_MAX;
+        if (act && !heavy) {
+            for (uint64_t e = beg; e < end; ++e) {
+                const uint32_t s = cnbrs[e];
+                ++my_scan;
+                if (in_bits && !bit_test(in_bits, s)) continue;
+                const uint32_t nv = relax_val<RELAX>(prop[s]);
+                if (nv < cur) {
+                    cur = nv;
+                    if (!dst_ok(cur, dkind, darg)) break;
+                }
+            }
+        }
+        uint64_t hm = __ballot(heavy);
+        while (hm) {
+            const int L = __ffsll((long long)hm) - 1;
+            hm &= hm - 1;
+            const uint64_t b = __shfl(beg, L, 64), en = __shfl(end, L, 64);
+            uint32_t best = __shfl(cur, L, 64);
+            for (uint64_t base = b; base < en; base += 64) {
+                const uint64_t e = base + ln;
+                uint32_t v = 0xffffffffu;
+                if (e < en) {
+                    const uint32_t s = cnbrs[e];
+                    if (!in_bits || bit_test(in_bits, s)) v = relax_val<RELAX>(prop[s]);
+                    ++my_scan;
+                }
+                v = wave_min(v);
+                best = v < best ? v : best;
+                if (!dst_ok(best, dkind, darg)) break;
+            }
+            if ((int)ln == L) cur = best;
+        }
+        const bool changed = valid && cur < orig;
+        if (changed) prop[d] = cur;
+        const bool rec = record == GPR_RECORD_REDUCTION ? changed
+                         : record == GPR_RECORD_METHOD_NE ? (valid && cur != prev[d])
+                                                          : false;
+        const uint64_t word = __ballot(rec);
+        if (out_bits && ln == 0) out_bits[grp] = word;
+        if (out_list && word) {
+            uint32_t base = 0;
+            if (ln == 0) base = atomicAdd(&ctr->listpos, (unsigned)__popcll(word));
+            base = __shfl(base, 0, 64);
+            if (rec) out_list[base + mbcnt(word)] = d;
+        }
+        if (ln == 0) my_v += (uint32_t)__popcll(word);
+        if (rec) my_e += deg[d];
+    }
+    const uint32_t tv = block_sum(my_v, red32);
+    const uint64_t te = block_sum(my_e, red);
+    const uint64_t ts = block_sum(my_scan, red);
+    if (threadIdx.x == 0) {
+        if (tv) atomicAdd(&ctr->nactv, tv);
+        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+        if (ts) atomicAdd(&ctr->edges, (unsigned long long)ts);
+    }
+}
+
+// ---- vertex maps -------------------------------------------------------------
+__global__ void k_vmap_fill32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_vmap_fill64(uint64_t *p, uint32_t n, uint64_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_vmap_iota(uint32_t *p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+__global__ void k_vmap_fill_list32(uint32_t *p, const uint32_t *__restrict__ list, uint32_t k, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) p[list[i]] = v;
+}
+__global__ void k_vmap_fill_bits32(uint32_t *p, const uint64_t *__restrict__ bits, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (bit_test(bits, i)) p[i] = v;
+}
+
+// vertex_filter over a bitmap or all-true frontier: out bit v = in(v) &&
+// ((p[v] == value) != neg).  One wave per output word.
+__global__ void __launch_bounds__(256)
+k_vfilter_bits(uint32_t n, const uint32_t *__restrict__ p, uint32_t value, int neg,
+               const uint64_t *__restrict__ in_bits, uint64_t *out_bits, const uint32_t *__restrict__ deg,
+               DevCounters *ctr) {
+    __shared__ uint64_t red[4];
+    __shared__ uint32_t red32[4];
+    const uint32_t ngroups = (n + 63) / 64;
+    const uint32_t wid = (blockIdx.x * 256 + threadIdx.x) / 64, nw = gridDim.x * 4;
+    uint32_t my_v = 0;
+    uint64_t my_e = 0;
+    for (uint32_t grp = wid; grp < ngroups; grp += nw) {
+        const uint32_t v = grp * 64 + lane();
+        bool keep = v < n && (!in_bits || bit_test(in_bits, v));
+        if (keep) keep = ((p[v] == value) != (neg != 0));
+        const uint64_t w = __ballot(keep);
+        if (lane() == 0) {
+            out_bits[grp] = w;
+            my_v += (uint32_t)__popcll(w);
+        }
+        if (keep) my_e += deg[v];
+    }
+    const uint32_t tv = block_sum(my_v, red32);
+    const uint64_t te = block_sum(my_e, red);
+    if (threadIdx.x == 0) {
+        if (tv) atomicAdd(&ctr->nactv, tv);
+        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__restrict__ p, uint32_t value,
+               int neg, uint32_t *out, const uint32_t *__restrict__ deg, DevCounters *ctr) {
+    __shared__ uint64_t red[4];
+    uint64_t my_e = 0;
+    const uint32_t stride = gridDim.x * 256;
+    for (uint32_t b = blockIdx.x * 256; b < k; b += stride) {
+        const uint32_t i = b + threadIdx.x;
+        uint32_t v = 0;
+        bool keep = false;
+        if (i < k) {
+            v = in[i];
+            keep = (p[v] == value) != (neg != 0);
+        }
+        const uint64_t w = __ballot(keep);
+        if (w) {
+            uint32_t base = 0;
+            if (lane() == 0) base = atomicAdd(&ctr->listpos, (unsigned)__popcll(w));
+            base = __shfl(base, 0, 64);
+            if (keep) out[base + mbcnt(w)] = v;
+        }
+        if (keep) my_e += deg[v];
+    }
+    const uint64_t te = block_sum(my_e, red);
+    if (threadIdx.x == 0 && te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+}
+
+// ---- host side ----------------------------------------------------------------
+static unsigned pull_grid(uint32_t n) {
+    uint32_t groups = (n + 63) / 64;
+    return grid_for(groups, PULL_THREADS / 64, 2048);
+}
+
+int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st);
+int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *out_kind);
+
+int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_frontier *out,
+                     gpr_iter_stats *st) {
+    GPR_REQUIRE(desc->prop && desc->prop->dtype == GPR_U32 && desc->prop->g == g,
+                "edgemap: relax needs a u32 property of this graph");
+    GPR_REQUIRE(desc->record == GPR_RECORD_NONE || out, "edgemap: record needs an output frontier");
+    GPR_REQUIRE(!out || out != in, "edgemap: output frontier must differ from the input");
+    GPR_REQUIRE(desc->record != GPR_RECORD_METHOD_NE || (desc->prev && desc->prev->dtype == GPR_U32),
+                "edgemap: record method needs a u32 prev property");
+    const uint32_t n = g->n;
+    const uint64_t m = g->m;
+    const int in_kind = in ? in->kind : GPR_FRONTIER_ALL;
+    const uint32_t nactv = in ? in->nactv : n;
+    const uint64_t nacte = in ? in->nacte : m;
+    gpr_iter_stats s = {};
+    s.nactv_in = nactv;
+    s.nacte_in = nacte;
+    if (in_kind == GPR_FRONTIER_EMPTY || nactv == 0) {
+        if (out) { out->kind = GPR_FRONTIER_EMPTY; out->nactv = 0; out->nacte = 0; }
+        if (st) *st = s;
+        return GPR_OK;
+    }
+    int dir = desc->direction;
+    int okind = desc->out_kind;
+    if (dir == GPR_DIR_POLICY) GPR_TRY(policy_decide(g, nactv, nacte, &dir, &okind));
+    if (dir == GPR_DIR_AUTO) {
+        const float pct = desc->threshold_pct > 0.f ? desc->threshold_pct : 5.f;
+        // frac_threshold::is_sparse, api.h:135-139
+        const bool sparse = (float)((uint64_t)nactv + nacte) <= (float)m / (100.0f / pct);
+        dir = sparse ? GPR_DIR_PUSH : GPR_DIR_PULL;
+    }
+    if (okind == 0) okind = dir == GPR_DIR_PUSH ? (nacte > n ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST)
+                                                : GPR_FRONTIER_BITMAP;
+    if (!out) okind = 0;
+    if (okind == GPR_FRONTIER_LIST) GPR_TRY(frontier_ensure_list(out));
+    if (okind == GPR_FRONTIER_BITMAP) GPR_TRY(frontier_ensure_bits(out));
+    uint32_t *prop = (uint32_t *)desc->prop->d;
+    const uint32_t *prev = desc->prev ? (const uint32_t *)desc->prev->d : nullptr;
+    GPR_REQUIRE(in || dir != GPR_DIR_PUSH,
+                "edgemap: push over the implicit all-true frontier needs an explicit frontier");
+    GPR_HIP(hipEventRecord(g->ev0, g->stream));
+    // representation changes first: they use the counters themselves
+    const uint64_t *in_bits = nullptr;
+    if (dir == GPR_DIR_PUSH && in_kind != GPR_FRONTIER_LIST) GPR_TRY(frontier_to_list(in));
+    if (dir == GPR_DIR_PULL && in_kind != GPR_FRONTIER_ALL) {
+        GPR_TRY(frontier_to_bits(in));
+        in_bits = in->bits;
+    }
+    GPR_TRY(reset_counters(g));
+    if (dir == GPR_DIR_PUSH) {
+        const uint32_t k = in->nactv;
+        const uint64_t E = in->nacte;
+        const uint32_t nb = (k + SCAN_BLOCK - 1) / SCAN_BLOCK;
+        ProfScope ps(g, "push");
+        hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, in->list, k, g->deg, g->scan,
+                           g->scan_blk);
+        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
+        uint64_t *bits = g->dedup;
+        if (okind == GPR_FRONTIER_BITMAP) {
+            GPR_TRY(fill_words(g, out->bits, nwords(n), 0ull));
+            bits = out->bits;
+        }
+        uint32_t *olist = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
+        if (E) {
+            const unsigned grid = grid_for(E, PUSH_TILE, 8192);
+            if (desc->relax == GPR_RELAX_MIN_PLUS1)
+                hipLaunchKernelGGL(k_push<GPR_RELAX_MIN_PLUS1>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
+                                   g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
+                                   desc->dst_arg, bits, olist, g->ctr);
+            else
+                hipLaunchKernelGGL(k_push<GPR_RELAX_MIN>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
+                                   g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
+                                   desc->dst_arg, bits, olist, g->ctr);
+        }
+        if (olist) GPR_TRY(clear_dedup_of_list(g, olist, E < n ? E : n));
+        s.edges_scanned = E;
+    } else {
+        uint64_t *ob = okind == GPR_FRONTIER_BITMAP ? out->bits : nullptr;
+        uint32_t *ol = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
+        int record = out ? desc->record : GPR_RECORD_NONE;
+        ProfScope ps(g, "pull");
+        if (desc->relax == GPR_RELAX_MIN_PLUS1)
+            hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN_PLUS1>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n,
+                               g->cindex, g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits,
+                               ob, ol, g->ctr);
+        else
+            hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex,
+                               g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol,
+                               g->ctr);
+    }
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipEventRecord(g->ev1, g->stream));
+    GPR_TRY(read_counters(g));
+    float ms = 0.f;
+    GPR_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    if (out) {
+        out->kind = okind;
+        out->nactv = okind == GPR_FRONTIER_LIST ? g->hctr->listpos : g->hctr->nactv;
+        out->nacte = g->hctr->nacte;
+        if (out->nactv == 0) out->kind = GPR_FRONTIER_EMPTY;
+        s.nactv_out = out->nactv;
+        s.nacte_out = out->nacte;
+    }
+    if (dir == GPR_DIR_PULL) s.edges_scanned = g->hctr->edges;
+    s.direction = dir;
+    s.out_kind = okind;
+    s.ms = ms;
+    if (st) *st = s;
+    return GPR_OK;
+}
+
+} // namespace gpr
+
+using namespace gpr;
+
+extern "C" int gpr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_frontier *out,
+                           gpr_iter_stats *st) {
+    if (!g || !desc) return GPR_E_INVAL;
+    GPR_REQUIRE(!in || in->g == g, "edgemap: input frontier belongs to another graph");
+    GPR_REQUIRE(!out || out->g == g, "edgemap: output frontier belongs to another graph");
+    switch (desc->relax) {
+    case GPR_RELAX_MIN_PLUS1:
+    case GPR_RELAX_MIN:
+        return edgemap_minrelax(g, desc, in, out, st);
+    case GPR_RELAX_PR_SUM:
+        return pr_edgemap(g, desc, in, st);
+    default:
+        set_error("edgemap: unknown relax %d", desc->relax);
+        return GPR_E_INVAL;
+    }
+}
+
+extern "C" int gpr_vertex_map(gpr_graph *g, int op, gpr_prop *p, const gpr_prop *q, gpr_frontier *f,
+                              uint64_t value_bits) {
+    if (!g || !p || p->g != g) return GPR_E_INVAL;
+    const uint32_t n = g->n;
+    const unsigned grid = grid_for(n, 256, 8192);
+    switch (op) {
+    case GPR_VMAP_FILL:
+        if (p->elem == 4)
+            hipLaunchKernelGGL(k_vmap_fill32, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d, n, (uint32_t)value_bits);
+        else if (p->elem == 8)
+            hipLaunchKernelGGL(k_vmap_fill64, dim3(grid), dim3(256), 0, g->stream, (uint64_t *)p->d, n, value_bits);
+        else {
+            set_error("vertex_map fill: unsupported element size %zu", p->elem);
+            return GPR_E_INVAL;
+        }
+        break;
+    case GPR_VMAP_COPY:
+        GPR_REQUIRE(q && q->elem == p->elem && q->g == g, "vertex_map copy: mismatched properties");
+        GPR_HIP(hipMemcpyAsync(p->d, q->d, p->elem * (size_t)n, hipMemcpyDeviceToDevice, g->stream));
+        break;
+    case GPR_VMAP_IOTA:
+        GPR_REQUIRE(p->dtype == GPR_U32, "vertex_map iota needs u32");
+        hipLaunchKernelGGL(k_vmap_iota, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d, n);
+        break;
+    case GPR_VMAP_FILL_FRONTIER:
+        GPR_REQUIRE(p->elem == 4 && f && f->g == g, "vertex_map over a frontier needs a u32 property");
+        if (f->kind == GPR_FRONTIER_ALL)
+            hipLaunchKernelGGL(k_vmap_fill32, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d, n, (uint32_t)value_bits);
+        else if (f->kind == GPR_FRONTIER_LIST && f->nactv)
+            hipLaunchKernelGGL(k_vmap_fill_list32, dim3(grid_for(f->nactv, 256, 8192)), dim3(256), 0, g->stream,
+                               (uint32_t *)p->d, f->list, f->nactv, (uint32_t)value_bits);
+        else if (f->kind == GPR_FRONTIER_BITMAP)
+            hipLaunchKernelGGL(k_vmap_fill_bits32, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d, f->bits, n,
+                               (uint32_t)value_bits);
+        break;
+    default:
+        set_error("vertex_map: unknown op %d", op);
+        return GPR_E_INVAL;
+    }
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+extern "C" int gpr_vertex_filter(gpr_graph *g, gpr_frontier *in, gpr_prop *p, uint32_t value, int neg,
+                                 gpr_frontier *out) {
+    if (!g || !p || !out || p->dtype != GPR_U32 || out == in) return GPR_E_INVAL;
+    const int kind = in ? in->kind : GPR_FRONTIER_ALL;
+    GPR_TRY(reset_counters(g));
+    if (kind == GPR_FRONTIER_EMPTY) {
+        out->kind = GPR_FRONTIER_EMPTY;
+        out->nactv = 0;
+        out->nacte = 0;
+        return GPR_OK;
+    }
+    if (kind == GPR_FRONTIER_LIST) {
+        GPR_TRY(frontier_ensure_list(out));
+        hipLaunchKernelGGL(k_vfilter_list, dim3(grid_for(in->nactv, 256, 4096)), dim3(256), 0, g->stream, in->list,
+                           in->nactv, (const uint32_t *)p->d, value, neg, out->list, g->deg, g->ctr);
+        GPR_HIP(hipGetLastError());
+        GPR_TRY(read_counters(g));
+        out->kind = GPR_FRONTIER_LIST;
+        out->nactv = g->hctr->listpos;
+        out->nacte = g->hctr->nacte;
+    } else {
+        GPR_TRY(frontier_ensure_bits(out));
+        hipLaunchKernelGGL(k_vfilter_bits, dim3(pull_grid(g->n)), dim3(256), 0, g->stream, g->n, (const uint32_t *)p->d,
+                           value, neg, kind == GPR_FRONTIER_BITMAP ? in->bits : nullptr, out->bits, g->deg, g->ctr);
+        GPR_HIP(hipGetLastError());
+        GPR_TRY(read_counters(g));
+        out->kind = GPR_FRONTIER_BITMAP;
+        out->nactv = g->hctr->nactv;
+        out->nacte = g->hctr->nacte;
+    }
+    if (out->nactv == 0) out->kind = GPR_FRONTIER_EMPTY;
+    return GPR_OK;
+}

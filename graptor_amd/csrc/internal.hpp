@@ -1,0 +1,193 @@
+// internal.hpp -- shared definitions of the gfx950 traversal engine.
+//
+// Data layout in HBM (per graph handle):
+//   index  u64[n+1]   CSR offsets (Graptor EID), nbrs u32[m] (Graptor VID)
+//   cindex/cnbrs      CSC; aliases the CSR for symmetric graphs
+//   deg    u32[n]     out-degree
+//   bucket u32[nb+1]  row starts of the pull "stream" buckets over the CSC:
+//                     rows r with floor((cindex[r] + r) / STREAM_ITEMS) == b
+//   long rows         CSC rows with in-degree > LONG_DEG, cut into
+//                     LONG_CHUNK-edge chunks processed by whole workgroups
+// Frontiers are either a u32 vertex list or a bitmap of u64 words (bit v of
+// word v/64), one wave64 ballot per word.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+#include <map>
+
+#include "../../include/gpr.h"
+
+namespace gpr {
+
+// ---- tunables (the autotuner may override the launch-time ones) ---------
+constexpr int WAVE = 64;
+constexpr int STREAM_ITEMS = 2048;  // rows + edges per pull bucket
+constexpr int STREAM_THREADS = 256;
+constexpr uint32_t LONG_DEG = 2048;   // rows above this are "long"
+constexpr uint32_t LONG_CHUNK = 8192; // edges per long-row chunk
+constexpr int PUSH_THREADS = 256;
+constexpr int PUSH_ITEMS = 4;         // edges per thread per push tile
+constexpr int SCAN_BLOCK = 1024;      // frontier entries per scan block
+
+// ---- error handling ------------------------------------------------------
+void set_error(const char *fmt, ...);
+#define GPR_HIP(expr)                                                       \
+    do {                                                                    \
+        hipError_t e_ = (expr);                                             \
+        if (e_ != hipSuccess) {                                             \
+            ::gpr::set_error("%s:%d: %s: %s", __FILE__, __LINE__, #expr,    \
+                             hipGetErrorString(e_));                        \
+            return GPR_E_HIP;                                               \
+        }                                                                   \
+    } while (0)
+#define GPR_TRY(expr)                                                       \
+    do {                                                                    \
+        int r_ = (expr);                                                    \
+        if (r_ != GPR_OK) return r_;                                        \
+    } while (0)
+#define GPR_REQUIRE(cond, ...)                                              \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            ::gpr::set_error(__VA_ARGS__);                                  \
+            return GPR_E_INVAL;                                             \
+        }                                                                   \
+    } while (0)
+
+int dev_alloc(void **p, size_t bytes, const char *what);
+void dev_free(void *p);
+
+// ---- device-resident counters of one handle ------------------------------
+struct DevCounters {
+    unsigned long long nacte;   // edges of the frontier being produced
+    unsigned long long edges;   // edges examined
+    unsigned int nactv;         // vertices of the frontier being produced
+    unsigned int listpos;       // append position of the list being produced
+    unsigned int ticket;        // last-workgroup tickets (reset by that wg)
+    unsigned int ticket2;
+};
+
+// Scalars of the PageRank iteration, kept on the device so that an
+// iteration needs no host round trip.
+struct PrScalars {
+    double sum_y;      // sum of pulled ranks
+    double w;          // (1 - sum_y) / n
+    double inv_s;      // 1 / sum(y + w)
+    double delta;      // L1 residual of the last update
+    double sum_x;      // sum of the normalised ranks
+    float max_ratio;   // max over v of y[v]/deg[v] (for the FP16 scale)
+    float enc_scale;   // contribution encode scale 2^k (FP16 only)
+    float dec_scale;   // decode scale 2^-k of the stored contributions
+    float pad;
+};
+
+struct Profile {
+    bool on = false;
+    std::map<std::string, std::pair<double, uint64_t>> acc;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+} // namespace gpr
+
+struct gpr_graph {
+    uint32_t n = 0;
+    uint64_t m = 0;
+    int symmetric = 1;
+    uint64_t *index = nullptr;
+    uint32_t *nbrs = nullptr;
+    uint64_t *cindex = nullptr;   // == index when symmetric
+    uint32_t *cnbrs = nullptr;
+    uint32_t *deg = nullptr;
+    // pull partitions over the CSC
+    uint32_t nbuckets = 0;
+    uint32_t *bucket = nullptr;     // nbuckets+1 row starts
+    uint32_t nlong = 0;
+    uint32_t *long_rows = nullptr;  // nlong ids, ascending
+    uint32_t *long_base = nullptr;  // nlong+1 chunk prefix
+    uint32_t nlong_chunks = 0;
+    uint32_t hdv = 0;               // highest-degree vertex
+    hipStream_t stream = nullptr;
+    // scratch
+    gpr::DevCounters *ctr = nullptr;     // device
+    gpr::DevCounters *hctr = nullptr;    // pinned host mirror
+    uint64_t *dedup = nullptr;           // n/64 words, all zero between steps
+    uint64_t *scan = nullptr;            // frontier degree prefix (n+1)
+    uint64_t *scan_blk = nullptr;        // per-scan-block totals
+    double *partial = nullptr;           // per-workgroup partial sums
+    size_t partial_cap = 0;
+    float *partial_f = nullptr;          // per-workgroup partial maxima
+    double *long_partial = nullptr;      // per long chunk
+    double *dsum_slices = nullptr;       // 64 slice sums of deterministic_sum
+    gpr::PrScalars *prs = nullptr;       // device
+    // trace of the last run
+    std::vector<gpr_iter_stats> trace;
+    gpr::Profile prof;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;    // per edgemap step
+    hipEvent_t rev0 = nullptr, rev1 = nullptr;  // per application run
+    // recycled frontier buffers (avoid hipMalloc per iteration)
+    std::vector<void *> pool_list, pool_bits;
+};
+
+struct gpr_prop {
+    gpr_graph *g = nullptr;
+    int dtype = GPR_U32;
+    size_t elem = 4;
+    void *d = nullptr;
+};
+
+struct gpr_frontier {
+    gpr_graph *g = nullptr;
+    int kind = GPR_FRONTIER_EMPTY;
+    uint32_t nactv = 0;
+    uint64_t nacte = 0;
+    uint32_t *list = nullptr;   // capacity n
+    uint64_t *bits = nullptr;   // (n+63)/64 words
+};
+
+namespace gpr {
+
+inline size_t nwords(uint32_t n) { return ((size_t)n + 63) / 64; }
+inline unsigned grid_for(uint64_t work, unsigned per_block, unsigned cap = 0x7fffffffu) {
+    uint64_t g = (work + per_block - 1) / per_block;
+    if (g == 0) g = 1;
+    return (unsigned)(g > cap ? cap : g);
+}
+
+// Profiling helpers: record events around a launch on the handle's stream.
+struct ProfScope {
+    gpr_graph *g;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(gpr_graph *g_, const char *n_) : g(g_), name(n_) {
+        if (g->prof.on) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, g->stream);
+        }
+    }
+    ~ProfScope() {
+        if (g->prof.on) {
+            (void)hipEventRecord(b, g->stream);
+            g->prof.pending.push_back({name, {a, b}});
+        }
+    }
+};
+void prof_flush(gpr_graph *g);
+
+// Host entry points shared between translation units.
+int frontier_ensure_list(gpr_frontier *f);
+int frontier_ensure_bits(gpr_frontier *f);
+int frontier_to_bits(gpr_frontier *f);            // convert LIST/ALL -> BITMAP
+int frontier_to_list(gpr_frontier *f);            // convert BITMAP/ALL -> LIST
+int fill_words(gpr_graph *g, uint64_t *p, size_t words, uint64_t v);
+int clear_dedup_of_list(gpr_graph *g, const uint32_t *list, uint64_t max_count);
+int read_counters(gpr_graph *g);                  // D2H of ctr, synchronises
+int reset_counters(gpr_graph *g);
+int ensure_partials(gpr_graph *g, size_t count);
+int graph_setup(gpr_graph *g);                    // deg, buckets, long rows
+int deterministic_sum(gpr_graph *g, const double *parts, uint32_t count,
+                      double *dst);               // device-side, async
+
+} // namespace gpr

@@ -1,0 +1,257 @@
+/*
+ * gpr.h -- C ABI of the MI355X (gfx950) edgemap/vertexmap traversal engine.
+ *
+ * Graptor's operator API is a set of C++ templates (include/graptor/api.h)
+ * with no FFI; its lambdas cannot cross a C boundary.  This ABI therefore
+ * exports the fixed operator set that BFSLVLv, CCv, PRv and PR_vary use,
+ * with plain pointers and sizes.  The header-only C++ layer
+ * include/gpr/api.hpp rebuilds Graptor's api::edgemap / api::filter /
+ * api::record / api::relax / api::config / make_lazy_executor names on top
+ * of it.  Every entry point returns GPR_OK (0) or a GPR_E_* code; the text of
+ * the last error of the calling thread is available from gpr_last_error.
+ *
+ * Ownership: handles own their device memory and are released with the
+ * matching *_destroy call.  Host buffers passed in are copied; host buffers
+ * passed out are filled before the call returns.  Calls on one graph handle
+ * are serialised on that handle's HIP stream and are not thread-safe per
+ * handle; separate handles may be used from separate threads.
+ */
+#ifndef GPR_H
+#define GPR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference aborts instead: GraphCSx.h:2227-2244,
+ *      api.h:2376-2377) ---------------------------------------------------- */
+enum {
+    GPR_OK = 0,
+    GPR_E_INVAL = 1,   /* bad argument or unsupported combination */
+    GPR_E_FORMAT = 2,  /* malformed graph file or CSR */
+    GPR_E_OOM = 3,     /* device or host allocation failed */
+    GPR_E_HIP = 4,     /* HIP runtime error */
+    GPR_E_IO = 5,      /* file IO error */
+    GPR_E_NODEV = 6    /* no gfx950 device */
+};
+
+typedef struct gpr_graph gpr_graph;
+typedef struct gpr_prop gpr_prop;
+typedef struct gpr_frontier gpr_frontier;
+
+/* Copy the calling thread's last error message into buf (NUL-terminated). */
+int gpr_last_error(char *buf, size_t len);
+/* Library version string, e.g. "gpr 0.1 gfx950". */
+const char *gpr_version(void);
+/* Select and initialise device `device` (0 = first).  Fails with
+ * GPR_E_NODEV when the device is not a gfx950. */
+int gpr_init(int device);
+
+/* ---- graphs: GraphCSx (graph/GraphCSx.h:293-2806) ------------------------ */
+/* Upload a CSR (index has n+1 entries, nbrs m).  For non-symmetric graphs
+ * the CSC is built on the device.  Replaces GraphCSx(index, edges) and
+ * build_degree (GraphCSx.h:2797-2805). */
+int gpr_graph_create(const uint64_t *index, const uint32_t *nbrs, uint32_t n,
+                     uint64_t m, int symmetric, gpr_graph **out);
+/* Read a b2 file (versions 2 and 3) -- GraphCSx::readFromBinaryFile,
+ * GraphCSx.h:2137-2264. */
+int gpr_graph_load_b2(const char *path, int symmetric, gpr_graph **out);
+/* Write a version-2 b2 file -- GraphCSx::writeToBinaryFile, :2009-2027. */
+int gpr_graph_save_b2(const gpr_graph *g, const char *path);
+/* Generate a cleaned (no self loops, symmetrised, sorted, deduplicated)
+ * RMAT graph on the device: 2^scale vertices, edge_factor*2^scale generated
+ * edges, quadrant probabilities a, b, c (d = 1-a-b-c), counter-based stream
+ * keyed by seed.  Replaces the CleanGraph/RmDupEdges/CvtToUndir pipeline
+ * (tools/CleanGraph.C:10-139, RmDupEdges.C:9-59, CvtToUndir.C:30-121). */
+int gpr_graph_generate_rmat(uint32_t scale, uint32_t edge_factor, double a,
+                            double b, double c, uint64_t seed, gpr_graph **out);
+int gpr_graph_info(const gpr_graph *g, uint32_t *n, uint64_t *m,
+                   int *symmetric);
+/* Copy the CSR back (index: n+1 entries, nbrs: m).  Either may be NULL. */
+int gpr_graph_download(const gpr_graph *g, uint64_t *index, uint32_t *nbrs);
+/* Highest out-degree vertex, ties to the lowest ID
+ * (GraphCSx::findHighestDegreeVertex, GraphCSx.h:2446-2451). */
+int gpr_graph_highest_degree_vertex(const gpr_graph *g, uint32_t *v);
+int gpr_graph_destroy(gpr_graph *g);
+
+/* ---- vertex properties: api::vertexprop (api/vertexprop.h:28-143) ------- */
+enum { GPR_U32 = 0, GPR_F32 = 1, GPR_F64 = 2, GPR_U64 = 3, GPR_U16 = 4 };
+int gpr_prop_create(gpr_graph *g, int dtype, gpr_prop **out);
+int gpr_prop_upload(gpr_prop *p, const void *host);
+int gpr_prop_download(const gpr_prop *p, void *host);
+int gpr_prop_destroy(gpr_prop *p);
+
+/* ---- frontiers: frontier (frontier/frontier.h:58-1179) ------------------ */
+enum {
+    GPR_FRONTIER_EMPTY = 0,
+    GPR_FRONTIER_ALL = 1,     /* frontier::all_true */
+    GPR_FRONTIER_LIST = 2,    /* ft_sparse: u32 vertex list */
+    GPR_FRONTIER_BITMAP = 3   /* ft_bit: one bit per vertex, u64 words */
+};
+int gpr_frontier_create(gpr_graph *g, gpr_frontier **out); /* empty */
+/* frontier::create(n, v, deg) */
+int gpr_frontier_set_single(gpr_frontier *f, uint32_t v);
+int gpr_frontier_set_all(gpr_frontier *f);
+int gpr_frontier_set_list(gpr_frontier *f, const uint32_t *vs, uint32_t k);
+/* frontier::sparse(n, deg(v), &edges[index[v]]): the neighbours of v
+ * (CCv.C:262-267, PRESET_ZERO). */
+int gpr_frontier_set_neighbours(gpr_frontier *f, uint32_t v);
+/* Active counts nActiveVertices / nActiveEdges (frontier.h:694-697) and
+ * representation (GPR_FRONTIER_*). */
+int gpr_frontier_stats(const gpr_frontier *f, uint32_t *nactv,
+                       uint64_t *nacte, int *kind);
+/* Convert in place: toSparse (frontier.h:576-634) / to a bitmap. */
+int gpr_frontier_convert(gpr_frontier *f, int kind);
+/* Copy the active vertices out, ascending.  *k receives the count; out
+ * must hold nactv entries. */
+int gpr_frontier_download(gpr_frontier *f, uint32_t *out, uint32_t *k);
+int gpr_frontier_destroy(gpr_frontier *f);
+
+/* ---- edgemap: api::edgemap (api.h:2143-2415) ----------------------------- */
+enum {                      /* relax operator, api::relax (api.h:1670-1687) */
+    GPR_RELAX_MIN_PLUS1 = 0, /* p[d].min(p[s]+1)   BFSLVLv.C:208-214 */
+    GPR_RELAX_MIN = 1,       /* p[d].min(p[s])     CCv.C:571-577 */
+    GPR_RELAX_PR_SUM = 2     /* y[d] += contrib[s] PRv.C:202-212 */
+};
+enum {                      /* destination filter, api::filter(dst) */
+    GPR_DST_NONE = 0,
+    GPR_DST_GT = 1,          /* p[d] >  arg   BFSLVLv.C:193-196 */
+    GPR_DST_NE = 2           /* p[d] != arg   CCv.C:559-564 */
+};
+enum {                      /* api::record (api.h:1450-1516) */
+    GPR_RECORD_NONE = 0,
+    GPR_RECORD_REDUCTION = 1, /* activate d when relax changed p[d] */
+    GPR_RECORD_METHOD_NE = 2  /* activate d when p[d] != prev[d] */
+};
+enum {                      /* traversal direction / threshold */
+    GPR_DIR_AUTO = 0,        /* threshold rule, api.h:130-160 */
+    GPR_DIR_PUSH = 1,        /* always_sparse */
+    GPR_DIR_PULL = 2,        /* always_dense */
+    GPR_DIR_POLICY = 3       /* learned policy (gpr_policy_*) */
+};
+enum {                      /* contribution storage for GPR_RELAX_PR_SUM */
+    GPR_CONTRIB_F32 = 0,     /* FP32 store, FP64 accumulate (PRv) */
+    GPR_CONTRIB_BF16 = 1,    /* bfloat16 RNE store, FP32 accumulate */
+    GPR_CONTRIB_F16 = 2,     /* IEEE half + per-iteration 2^k scale */
+    GPR_CONTRIB_CFP16 = 3,   /* customfp<6,10> truncating (customfp.h) */
+    GPR_CONTRIB_BF16T = 4    /* scustomfp<true,8,7>: bf16 truncating */
+};
+
+typedef struct {
+    int relax;            /* GPR_RELAX_* */
+    gpr_prop *prop;       /* relaxed property (u32) */
+    gpr_prop *prev;       /* previous copy for GPR_RECORD_METHOD_NE */
+    int src_strong;       /* api::filter(strong|weak, src, F) */
+    int dst_filter;       /* GPR_DST_* */
+    uint32_t dst_arg;
+    int record;           /* GPR_RECORD_* */
+    int direction;        /* GPR_DIR_* */
+    int out_kind;         /* 0 = auto, GPR_FRONTIER_LIST / _BITMAP */
+    float threshold_pct;  /* frac_threshold(t); 5 = default m/20 */
+    /* GPR_RELAX_PR_SUM only: prop is the f32 accumulator y, contrib the
+     * stored contributions in format contrib_fmt (GPR_CONTRIB_*). */
+    gpr_prop *contrib;
+    int contrib_fmt;
+    float damping;        /* 0.85 when 0 */
+} gpr_emap_desc;
+
+typedef struct {
+    uint32_t nactv_in;
+    uint64_t nacte_in;
+    uint32_t nactv_out;
+    uint64_t nacte_out;
+    int direction;        /* GPR_DIR_PUSH or GPR_DIR_PULL as executed */
+    int out_kind;         /* GPR_FRONTIER_* produced */
+    uint64_t edges_scanned; /* edges examined (upper bound for pull) */
+    double ms;            /* device time of the step */
+} gpr_iter_stats;
+
+/* One edgemap step over frontier `in`, recording into `out` (which may be
+ * NULL when record is GPR_RECORD_NONE).  `in` may be NULL for all-true. */
+int gpr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in,
+                gpr_frontier *out, gpr_iter_stats *st);
+
+/* ---- vertexmap: make_lazy_executor().vertex_map/scan/filter
+ *      (dsl/vertexmap.h:2098-2456) ----------------------------------------- */
+enum {
+    GPR_VMAP_FILL = 0,       /* p[v] = value */
+    GPR_VMAP_COPY = 1,       /* p[v] = q[v]   (maintain_copies :2515-2535) */
+    GPR_VMAP_IOTA = 2,       /* p[v] = v */
+    GPR_VMAP_FILL_FRONTIER = 3 /* p[v] = value for v in F */
+};
+int gpr_vertex_map(gpr_graph *g, int op, gpr_prop *p, const gpr_prop *q,
+                   gpr_frontier *f, uint64_t value_bits);
+/* vertex_filter(G, F, out, p[v] == value) or != value (neg=1)
+ * (CCv.C:481-509 find_zeros / find_nonzeros). */
+int gpr_vertex_filter(gpr_graph *g, gpr_frontier *in, gpr_prop *p,
+                      uint32_t value, int neg, gpr_frontier *out);
+
+/* PageRank vertex maps.  gpr_pr_contrib: contrib[v] = damping*x[v]/deg(v)
+ * in format fmt (PRv.C:176-187, the MEMO contribution array).
+ * gpr_pr_finish: the fused tail of one iteration after a GPR_RELAX_PR_SUM
+ * edgemap into y (PRv.C:214-269): y += (1-sum y)/n; y *= 1/sum y;
+ * delta = sum |x-y|; x = y; contrib = damping*x/deg.  The sums are FP64 and
+ * deterministic.  delta and sum_x are returned when non-NULL (this
+ * synchronises); otherwise they stay on the device (gpr_pr_scalars). */
+int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib,
+                   int fmt, float damping);
+int gpr_pr_finish(gpr_graph *g, gpr_prop *x, const gpr_prop *y,
+                  gpr_prop *contrib, int fmt, float damping, double *delta,
+                  double *sum_x);
+int gpr_pr_scalars(gpr_graph *g, double *sum_y, double *delta, double *sum_x,
+                   float *enc_scale);
+
+/* ---- applications: the callers of the hot path -------------------------- */
+typedef struct {
+    int direction;        /* GPR_DIR_* (AUTO = Graptor threshold) */
+    int out_kind;         /* 0 = auto */
+    float threshold_pct;  /* 5 = m/20 */
+    int max_iter;         /* PR: -pr:maxiter (default 100) */
+    double tol;           /* PR: 1e-7 */
+    int trace;            /* record per-iteration stats (-itimes) */
+} gpr_run_opts;
+
+typedef struct {
+    int iters;
+    int push_iters;
+    uint64_t edges;       /* edges traversed for GTEPS */
+    double ms;            /* wall time of run() on the device stream */
+    double delta;         /* PR: final L1 residual */
+    double sum;           /* PR: sum of ranks */
+} gpr_run_info;
+
+/* BFSLVLv (bench/BFSLVLv.C:110-254): levels[n], unreached = n+1. */
+int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *o,
+               uint32_t *levels, gpr_run_info *info);
+/* CCv (bench/CCv.C:221-588): labels[n] = min vertex ID of the component
+ * (the recode canonical form, CCv.C:121-169). */
+int gpr_ccv(gpr_graph *g, const gpr_run_opts *o, uint32_t *labels,
+            gpr_run_info *info);
+/* PRv (bench/PRv.C:115-290): FP32 ranks, FP64 accumulation. */
+int gpr_prv(gpr_graph *g, const gpr_run_opts *o, float *x,
+            gpr_run_info *info);
+/* PR_vary (bench/PR_vary.C:247-757): stages of (contribution format,
+ * tolerance) run in order, each continuing from the previous ranks, all
+ * sharing max_iter.  One stage with GPR_CONTRIB_F32 equals gpr_prv. */
+typedef struct { int contrib; double tol; } gpr_pr_stage;
+int gpr_pr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages,
+                const gpr_run_opts *o, float *x, gpr_run_info *info);
+
+/* Per-iteration trace of the last run (-itimes, BFSLVLv.C:90-106). */
+int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
+
+/* Device-event timing of named kernels over the runs since the last reset,
+ * measured on the handle's own stream.  name: "pr_pull", "pr_update",
+ * "push", "pull", ... */
+int gpr_profile_enable(gpr_graph *g, int on);
+int gpr_profile_get(const gpr_graph *g, const char *name, double *ms,
+                    uint64_t *launches);
+int gpr_profile_reset(gpr_graph *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPR_H */
