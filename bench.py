@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 traversal engine on RMAT-26 (one MI355X).
+
+One step = one BFSLVLv run (source: highest-degree vertex) + one CCv run +
+one PRv run of 20 FP32 power iterations, on a symmetric RMAT graph of scale
+26, edge factor 16 (a=.57 b=.19 c=.19 d=.05), generated and cleaned on the
+GPU before timing.  `value` is whole-job GTEPS: (BFS edges of the reached
+component + m for CC + 20*m for PR) / wall time of the K timed steps.
+
+Also reported: per-algorithm GTEPS, the HBM roofline of the dominant kernel
+(the PR pull, timed with HIP events on the engine's own stream), BF16/FP16
+PR per-iteration GTEPS, and the CPU baseline (the OpenMP restatement of
+Graptor's algorithm in oracle/, timed on the host on a bounded sample).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS (BFS, CC, PR iter) on RMAT-26, 1 GPU; % HBM roofline; vs host CPU"
+HBM_PEAK_GBS = 8000.0
+SEED = 0x47524150
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pr_pull_bytes(n, m, s):
+    """Algorithmic bytes of one PR pull launch: CSC offsets 8(n+1), neighbour
+    IDs 4m, one contribution gather of s bytes per edge, the y write 4n."""
+    return 8 * (n + 1) + 4 * m + s * m + 4 * n
+
+
+def run(args):
+    import numpy as np
+    import graptor_amd as G
+
+    t0 = time.time()
+    g = G.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, SEED)
+    n, m = g.n, g.m
+    src = g.highest_degree_vertex()
+    log("graph scale %d: n=%d m=%d source=%d (%.1fs)" % (args.scale, n, m, src, time.time() - t0))
+    if not args.no_reorder:
+        # preprocessing outside the timed region, as Graptor's VEBO
+        # relabelling is (driver.C:209-216 before the rounds loop)
+        t0 = time.time()
+        g.reorder()
+        log("degree-sorted relabelling (%.1fs)" % (time.time() - t0))
+
+    def step():
+        lv, ib = g.bfslvl(src)
+        lab, ic = g.ccv()
+        x, ip = g.prv(max_iter=args.pr_iters, tol=0.0)
+        return (lv, ib), (lab, ic), (x, ip)
+
+    for w in range(args.warmup):
+        step()
+        log("warmup %d done" % w)
+    g.profile(True)
+    g.profile_reset()
+    tb = tc = tp = 0.0
+    eb = ec = ep = 0
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        (lv, ib), (lab, ic), (x, ip) = step()
+        tb += ib.ms
+        tc += ic.ms
+        tp += ip.ms
+        eb += ib.edges
+        ec += ic.edges
+        ep += ip.edges
+        log("step %d: bfs %.2f ms (%d iters), cc %.2f ms (%d iters), pr %.2f ms" % (k, ib.ms, ib.iters, ic.ms, ic.iters,
+                                                                                 ip.ms))
+    wall = time.perf_counter() - t_start
+    pull_ms, pull_launches = g.profile_get("pr_pull")
+    g.profile(False)
+    total_edges = eb + ec + ep
+    value = total_edges / wall / 1e9
+    avg_pull_ms = pull_ms / max(1, pull_launches)
+    bytes_pull = pr_pull_bytes(n, m, 4)
+    achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
+    gteps = {
+        "bfs": eb / (tb * 1e-3) / 1e9,
+        "cc": ec / (tc * 1e-3) / 1e9,
+        "pr_iter_fp32": ep / (tp * 1e-3) / 1e9,
+    }
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (BFS levels, CC labels); f32 store / f64 accumulate (PR)",
+        "data": "synthetic RMAT generated on the GPU from a fixed seed",
+        "config": {
+            "workload": "RMAT-%d ef%d a=.57 b=.19 c=.19 d=.05 symmetric, cleaned: BFSLVLv from the highest-degree "
+                        "vertex + CCv + PRv %d FP32 iterations" % (args.scale, args.edge_factor, args.pr_iters),
+            "scale": args.scale, "edge_factor": args.edge_factor, "n": n, "m": m, "seed": SEED,
+            "bfs_source": src, "parallelism": "single GPU",
+            "vertex_order": "original" if args.no_reorder else "degree-sorted relabelling (untimed preprocessing)",
+        },
+        "gteps": gteps,
+        "device_ms": {"bfs": tb / args.steps, "cc": tc / args.steps, "pr": tp / args.steps},
+        "roofline": {
+            "kernel": "k_pr_pull<fp32>",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_pull,
+            "avg_launch_ms": avg_pull_ms,
+            "launches": pull_launches,
+        },
+    }
+    # reduced-precision PR, per iteration (PR_vary contribution formats)
+    if not args.no_precision:
+        prec = {}
+        for fmt, s in (("bf16", 2), ("fp16", 2), ("cfp16", 2)):
+            g.profile(True)
+            g.profile_reset()
+            g.pr_vary([(fmt, 0.0)], max_iter=args.pr_iters)
+            xr, info = g.pr_vary([(fmt, 0.0)], max_iter=args.pr_iters)
+            pm, pl = g.profile_get("pr_pull")
+            g.profile(False)
+            prec[fmt] = {"gteps_iter": info.edges / (info.ms * 1e-3) / 1e9,
+                         "pull_avg_ms": pm / max(1, pl),
+                         "pull_GBs": pr_pull_bytes(n, m, s) / (pm / max(1, pl) * 1e-3) / 1e9,
+                         "speedup_vs_fp32_iter": (info.edges / info.ms) / (ep / tp)}
+        res["pr_precision"] = prec
+    if not args.no_cpu:
+        res["cpu_baseline"], res["parity"] = cpu_baseline(g, args, src, lv, lab, x)
+    g.destroy()
+    return res
+
+
+def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu):
+    """The OpenMP restatement of Graptor's algorithm (oracle/) on the host
+    cores: one BFSLVLv run, one CCv run and a few PRv iterations on the same
+    graph.  Its results double as a full-size parity check of the GPU run."""
+    import numpy as np
+    import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    t0 = time.time()
+    # the CPU runs on the same vertex layout as the GPU (internal IDs)
+    index, nbrs = g.download()
+    order = g.order()
+    log("downloaded graph for the CPU baseline (%.1fs)" % (time.time() - t0))
+    n, m = len(index) - 1, len(nbrs)
+    src_int = g.remap_id(src)
+    t = time.perf_counter()
+    lv, bi = O.graptor_bfs(index, nbrs, src_int, threads)
+    t_bfs = time.perf_counter() - t
+    deg = np.diff(index)
+    e_bfs = int(deg[lv <= n].sum())
+    log("cpu bfs %.2fs" % t_bfs)
+    t = time.perf_counter()
+    lab, ci = O.graptor_cc(index, nbrs, threads)
+    t_cc = time.perf_counter() - t
+    log("cpu cc %.2fs" % t_cc)
+    pr_it = args.cpu_pr_iters
+    t = time.perf_counter()
+    xc, it, _ = O.graptor_pr(index, nbrs, pr_it, 0.0, threads)
+    t_pr = time.perf_counter() - t
+    log("cpu pr %d iters %.2fs" % (pr_it, t_pr))
+    tot_e = e_bfs + m + pr_it * m
+    tot_t = t_bfs + t_cc + t_pr
+    base = {
+        "value": tot_e / tot_t / 1e9,
+        "unit": "GTEPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": "one BFSLVLv run + one CCv run + %d PRv iterations on the same RMAT-%d graph "
+                  "(OpenMP restatement of Graptor's push/pull engine, oracle/oracle.c)" % (pr_it, args.scale),
+        "gteps": {"bfs": e_bfs / t_bfs / 1e9, "cc": m / t_cc / 1e9, "pr_iter_fp32": pr_it * m / t_pr / 1e9},
+        "seconds": {"bfs": t_bfs, "cc": t_cc, "pr": t_pr},
+        "cpu": _cpu_model(),
+    }
+    # CPU results are in internal IDs: bring the GPU's (original IDs) over
+    # and canonicalise the CPU labels to the minimum original ID per
+    # component (CCv.C:121-169)
+    minorig = np.full(n, np.iinfo(np.uint32).max, np.uint32)
+    np.minimum.at(minorig, lab, order)
+    parity = {"bfs_levels_equal": bool(np.array_equal(lv, lv_gpu[order])),
+              "cc_labels_equal": bool(np.array_equal(minorig[lab], lab_gpu[order]))}
+    return base, parity
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--pr-iters", type=int, default=20)
+    ap.add_argument("--cpu-pr-iters", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-reorder", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in the "
+              "documentation); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    res = run(args)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -80,6 +80,11 @@ SIGNATURES = {
                                           C.c_uint64, _PP]),
     "gpr_graph_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
     "gpr_graph_download": (C.c_int, [_P, _P, _P]),
+    "gpr_graph_reorder": (C.c_int, [_P, C.c_int]),
+    "gpr_graph_order": (C.c_int, [_P, _P]),
+    "gpr_graph_remap_id": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "gpr_graph_original_id": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "gpr_cc_recode": (C.c_int, [_P, _P]),
     "gpr_graph_highest_degree_vertex": (C.c_int, [_P, C.POINTER(C.c_uint32)]),
     "gpr_graph_destroy": (C.c_int, [_P]),
     "gpr_prop_create": (C.c_int, [_P, C.c_int, _PP]),
@@ -194,10 +199,33 @@ class Graph:
         _check(lib().gpr_graph_save_b2(self.h, path.encode()))
 
     def download(self):
+        """CSR in internal IDs (see order())."""
         index = np.empty(self.n + 1, np.uint64)
         nbrs = np.empty(max(1, self.m), np.uint32)
         _check(lib().gpr_graph_download(self.h, _ptr(index), _ptr(nbrs)))
         return index, nbrs[: self.m]
+
+    def reorder(self, mode=1):
+        """Relabel vertices internally by decreasing degree (IDs at the API
+        stay original)."""
+        _check(lib().gpr_graph_reorder(self.h, mode))
+        return self
+
+    def order(self):
+        """order[i] = original ID of internal vertex i."""
+        o = np.empty(max(1, self.n), np.uint32)
+        _check(lib().gpr_graph_order(self.h, _ptr(o)))
+        return o[: self.n]
+
+    def remap_id(self, v):
+        r = C.c_uint32()
+        _check(lib().gpr_graph_remap_id(self.h, v, C.byref(r)))
+        return r.value
+
+    def original_id(self, v):
+        r = C.c_uint32()
+        _check(lib().gpr_graph_original_id(self.h, v, C.byref(r)))
+        return r.value
 
     def highest_degree_vertex(self):
         v = C.c_uint32()

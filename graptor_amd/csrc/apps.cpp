@@ -141,10 +141,13 @@ struct CCv {
         A::make_lazy_executor(GA).vertex_map_iota(m_IDs).materialize();
         iter = 0;
         uint32_t initial_zeros = 1;
-        // PRESET_ZERO, CCv.C:262-291: IDs[N(0)] = 0
+        // PRESET_ZERO, CCv.C:262-291: IDs[N(0)] = 0, vertex 0 of the
+        // (relabelled) graph, i.e. the highest-degree vertex after reordering
         if (n > 0) {
+            uint32_t v0 = 0;
+            check(gpr_graph_original_id(GA.get(), 0, &v0), "original id");
             A::frontier G0 = A::frontier::empty(GA);
-            check(gpr_frontier_set_neighbours(G0.get(), 0), "neighbours of 0");
+            check(gpr_frontier_set_neighbours(G0.get(), v0), "neighbours of 0");
             A::make_lazy_executor(GA).vertex_map_fill(G0, m_IDs, 0).materialize();
             initial_zeros += G0.nActiveVertices();
             G0.del();
@@ -292,7 +295,7 @@ extern "C" int gpr_ccv(gpr_graph *g, const gpr_run_opts *opts, uint32_t *labels,
         stream_time_ms(g, true);
         app.run(&local);
         local.ms = stream_time_ms(g, false);
-        if (labels) app.m_IDs.download(labels);
+        if (labels) check(gpr_cc_recode(app.m_IDs.get(), labels), "cc recode");
         if (info) *info = local;
     });
 }

@@ -201,6 +201,11 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
 // ---- pull ------------------------------------------------------------------
 constexpr int PULL_THREADS = 256;
 constexpr uint32_t PULL_SERIAL_MAX = 32;
+// The destination filter usually stops a row after a few edges (early
+// exit), so the lane-serial path loads one neighbour at a time: loading
+// ahead costs more frontier-bitmap lines than it hides latency.
+constexpr int PULL_UNROLL = 1;
+constexpr int PULL_HEAVY_UNROLL = 2;
 
 template <int RELAX>
 __global__ void __launch_bounds__(PULL_THREADS)
@@ -229,14 +234,28 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         }
         const bool heavy = act && (end - beg) > PULL_SERIAL_MAX;
         if (act && !heavy) {
-            for (uint64_t e = beg; e < end; ++e) {
-                const uint32_t s = cnbrs[e];
-                ++my_scan;
-                if (in_bits && !bit_test(in_bits, s)) continue;
-                const uint32_t nv = relax_val<RELAX>(prop[s]);
-                if (nv < cur) {
-                    cur = nv;
-                    if (!dst_ok(cur, dkind, darg)) break;
+            // PULL_UNROLL independent neighbour loads (and frontier tests)
+            // in flight per lane; edges past an early exit are loaded but
+            // not applied
+            bool live = true;
+            for (uint64_t e = beg; live && e < end; e += PULL_UNROLL) {
+                uint32_t s[PULL_UNROLL];
+                bool hit[PULL_UNROLL];
+#pragma unroll
+                for (int u = 0; u < PULL_UNROLL; ++u) s[u] = e + u < end ? cnbrs[e + u] : 0xffffffffu;
+#pragma unroll
+                for (int u = 0; u < PULL_UNROLL; ++u)
+                    hit[u] = s[u] != 0xffffffffu && (!in_bits || bit_test(in_bits, s[u]));
+#pragma unroll
+                for (int u = 0; u < PULL_UNROLL; ++u) {
+                    if (live && hit[u]) {
+                        const uint32_t nv = relax_val<RELAX>(prop[s[u]]);
+                        if (nv < cur) {
+                            cur = nv;
+                            if (!dst_ok(cur, dkind, darg)) live = false;
+                        }
+                    }
+                    if (live && s[u] != 0xffffffffu) ++my_scan;
                 }
             }
         }
@@ -246,13 +265,23 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
             hm &= hm - 1;
             const uint64_t b = __shfl(beg, L, 64), en = __shfl(end, L, 64);
             uint32_t best = __shfl(cur, L, 64);
-            for (uint64_t base = b; base < en; base += 64) {
-                const uint64_t e = base + ln;
+            for (uint64_t base = b; base < en; base += 64 * PULL_HEAVY_UNROLL) {
+                uint32_t s[PULL_HEAVY_UNROLL];
+#pragma unroll
+                for (int u = 0; u < PULL_HEAVY_UNROLL; ++u) {
+                    const uint64_t e = base + (uint64_t)u * 64 + ln;
+                    s[u] = e < en ? cnbrs[e] : 0xffffffffu;
+                }
                 uint32_t v = 0xffffffffu;
-                if (e < en) {
-                    const uint32_t s = cnbrs[e];
-                    if (!in_bits || bit_test(in_bits, s)) v = relax_val<RELAX>(prop[s]);
-                    ++my_scan;
+#pragma unroll
+                for (int u = 0; u < PULL_HEAVY_UNROLL; ++u) {
+                    if (s[u] != 0xffffffffu) {
+                        ++my_scan;
+                        if (!in_bits || bit_test(in_bits, s[u])) {
+                            const uint32_t nv = relax_val<RELAX>(prop[s[u]]);
+                            v = nv < v ? nv : v;
+                        }
+                    }
                 }
                 v = wave_min(v);
                 best = v < best ? v : best;

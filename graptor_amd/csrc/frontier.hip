@@ -247,6 +247,7 @@ extern "C" int gpr_frontier_set_neighbours(gpr_frontier *f, uint32_t v) {
     if (!f) return GPR_E_INVAL;
     gpr_graph *g = f->g;
     GPR_REQUIRE(v < g->n, "vertex %u out of range", v);
+    GPR_TRY(to_internal_id(g, v, &v));
     uint64_t ix[2];
     GPR_HIP(hipMemcpyAsync(ix, g->index + v, sizeof(ix), hipMemcpyDeviceToHost, g->stream));
     GPR_HIP(hipStreamSynchronize(g->stream));
@@ -278,6 +279,7 @@ extern "C" int gpr_frontier_set_list(gpr_frontier *f, const uint32_t *vs, uint32
     for (uint32_t i = 0; i < k; ++i) GPR_REQUIRE(vs[i] < g->n, "frontier vertex %u out of range", vs[i]);
     GPR_TRY(frontier_ensure_list(f));
     if (k) GPR_HIP(hipMemcpyAsync(f->list, vs, sizeof(uint32_t) * k, hipMemcpyHostToDevice, g->stream));
+    GPR_TRY(map_ids(g, f->list, k, true));
     GPR_TRY(reset_counters(g));
     if (k) hipLaunchKernelGGL(k_list_edges, dim3(grid_for(k, 256, 1024)), dim3(256), 0, g->stream, f->list, k, g->deg, g->ctr);
     GPR_HIP(hipGetLastError());
@@ -320,7 +322,14 @@ extern "C" int gpr_frontier_download(gpr_frontier *f, uint32_t *out, uint32_t *k
     }
     *k = f->nactv;
     if (out) {
-        GPR_HIP(hipMemcpyAsync(out, f->list, sizeof(uint32_t) * f->nactv, hipMemcpyDeviceToHost, g->stream));
+        const uint32_t *src = f->list;
+        if (g->order) {
+            uint32_t *tmp = (uint32_t *)g->scan;
+            GPR_HIP(hipMemcpyAsync(tmp, f->list, sizeof(uint32_t) * f->nactv, hipMemcpyDeviceToDevice, g->stream));
+            GPR_TRY(map_ids(g, tmp, f->nactv, false));
+            src = tmp;
+        }
+        GPR_HIP(hipMemcpyAsync(out, src, sizeof(uint32_t) * f->nactv, hipMemcpyDeviceToHost, g->stream));
         GPR_HIP(hipStreamSynchronize(g->stream));
         std::sort(out, out + f->nactv);
     }

@@ -117,6 +117,8 @@ static void graph_free(gpr_graph *g) {
     dev_free(g->index);
     dev_free(g->nbrs);
     dev_free(g->deg);
+    dev_free(g->order);
+    dev_free(g->perm);
     dev_free(g->bucket);
     dev_free(g->long_rows);
     dev_free(g->long_base);
@@ -192,6 +194,99 @@ int graph_new_device(uint32_t n, uint64_t m, gpr_graph **out) {
 }
 
 void graph_release(gpr_graph *g) { graph_free(g); }
+
+// Install a new (relabelled, symmetric) CSR in place of the current one and
+// rebuild the degree array and pull partitions; scratch stays.
+int graph_replace_csr(gpr_graph *g, uint64_t *index, uint32_t *nbrs) {
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    if (g->cindex != g->index) dev_free(g->cindex);
+    if (g->cnbrs != g->nbrs) dev_free(g->cnbrs);
+    dev_free(g->index);
+    dev_free(g->nbrs);
+    dev_free(g->bucket);
+    dev_free(g->long_rows);
+    dev_free(g->long_base);
+    dev_free(g->long_partial);
+    g->bucket = nullptr;
+    g->long_rows = g->long_base = nullptr;
+    g->long_partial = nullptr;
+    g->index = g->cindex = index;
+    g->nbrs = g->cnbrs = nbrs;
+    hipLaunchKernelGGL(k_degree, dim3(grid_for(g->n, 256, 8192)), dim3(256), 0, g->stream, g->index, g->n, g->deg);
+    GPR_HIP(hipGetLastError());
+    std::vector<uint64_t> h((size_t)g->n + 1);
+    GPR_HIP(hipMemcpyAsync(h.data(), g->index, sizeof(uint64_t) * h.size(), hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return build_partitions(g, h.data());
+}
+
+template <class T> __global__ void k_gather_pos(const T *__restrict__ src, const uint32_t *__restrict__ order, uint32_t n, T *dst) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[i] = src[order[i]];
+}
+template <class T> __global__ void k_scatter_pos(const T *__restrict__ src, const uint32_t *__restrict__ order, uint32_t n, T *dst) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) dst[order[i]] = src[i];
+}
+__global__ void k_map_ids(uint32_t *ids, uint32_t k, const uint32_t *__restrict__ map) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) ids[i] = map[ids[i]];
+}
+
+int map_ids(gpr_graph *g, uint32_t *d_ids, uint32_t k, bool to_internal) {
+    if (!g->order || k == 0) return GPR_OK;
+    hipLaunchKernelGGL(k_map_ids, dim3(grid_for(k, 256, 8192)), dim3(256), 0, g->stream, d_ids, k,
+                       to_internal ? g->perm : g->order);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+int to_internal_id(gpr_graph *g, uint32_t orig, uint32_t *internal) {
+    if (!g->order) {
+        *internal = orig;
+        return GPR_OK;
+    }
+    GPR_HIP(hipMemcpyAsync(internal, g->perm + orig, 4, hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return GPR_OK;
+}
+
+// Move a per-vertex array between original and internal positions through
+// the scan scratch (8(n+1) bytes, free between edgemap steps).
+static int permute_positions(gpr_graph *g, const void *src, void *dst, size_t elem, bool to_internal) {
+    const unsigned grid = grid_for(g->n, 256, 8192);
+#define PERM_CASE(T)                                                                                          \
+    if (to_internal)                                                                                          \
+        hipLaunchKernelGGL(k_gather_pos<T>, dim3(grid), dim3(256), 0, g->stream, (const T *)src, g->order, g->n, (T *)dst); \
+    else                                                                                                      \
+        hipLaunchKernelGGL(k_scatter_pos<T>, dim3(grid), dim3(256), 0, g->stream, (const T *)src, g->order, g->n, (T *)dst);
+    if (elem == 2) { PERM_CASE(uint16_t) }
+    else if (elem == 4) { PERM_CASE(uint32_t) }
+    else if (elem == 8) { PERM_CASE(uint64_t) }
+    else {
+        set_error("permute: element size %zu", elem);
+        return GPR_E_INVAL;
+    }
+#undef PERM_CASE
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+// Giant components send most vertices to one address: read first and only
+// take the atomic when it can lower the value (the running minimum drops
+// quickly, so almost every vertex skips it).
+__global__ void k_recode_min(const uint32_t *__restrict__ label, const uint32_t *__restrict__ order, uint32_t n,
+                             uint32_t *minorig) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t l = label[i], o = order ? order[i] : i;
+        if (o < __hip_atomic_load(&minorig[l], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) atomicMin(&minorig[l], o);
+    }
+}
+__global__ void k_recode_out(const uint32_t *__restrict__ label, const uint32_t *__restrict__ order, uint32_t n,
+                             const uint32_t *__restrict__ minorig, uint32_t *out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        out[order ? order[i] : i] = minorig[label[i]];
+}
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
 
 } // namespace gpr
 
@@ -361,15 +456,72 @@ extern "C" int gpr_prop_create(gpr_graph *g, int dtype, gpr_prop **out) {
 
 extern "C" int gpr_prop_upload(gpr_prop *p, const void *host) {
     if (!p || !host) return GPR_E_INVAL;
-    GPR_HIP(hipMemcpyAsync(p->d, host, p->elem * p->g->n, hipMemcpyHostToDevice, p->g->stream));
-    GPR_HIP(hipStreamSynchronize(p->g->stream));
+    gpr_graph *g = p->g;
+    if (g->order) {
+        GPR_HIP(hipMemcpyAsync(g->scan, host, p->elem * g->n, hipMemcpyHostToDevice, g->stream));
+        GPR_TRY(permute_positions(g, g->scan, p->d, p->elem, true));
+    } else {
+        GPR_HIP(hipMemcpyAsync(p->d, host, p->elem * g->n, hipMemcpyHostToDevice, g->stream));
+    }
+    GPR_HIP(hipStreamSynchronize(g->stream));
     return GPR_OK;
 }
 
 extern "C" int gpr_prop_download(const gpr_prop *p, void *host) {
     if (!p || !host) return GPR_E_INVAL;
-    GPR_HIP(hipMemcpyAsync(host, p->d, p->elem * p->g->n, hipMemcpyDeviceToHost, p->g->stream));
-    GPR_HIP(hipStreamSynchronize(p->g->stream));
+    gpr_graph *g = p->g;
+    if (g->order) {
+        GPR_TRY(permute_positions(g, p->d, g->scan, p->elem, false));
+        GPR_HIP(hipMemcpyAsync(host, g->scan, p->elem * g->n, hipMemcpyDeviceToHost, g->stream));
+    } else {
+        GPR_HIP(hipMemcpyAsync(host, p->d, p->elem * g->n, hipMemcpyDeviceToHost, g->stream));
+    }
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return GPR_OK;
+}
+
+// CCv recode (bench/CCv.C:121-169): every vertex gets the minimum original
+// vertex ID of its component; labels[] is indexed by original ID.
+extern "C" int gpr_cc_recode(const gpr_prop *p, uint32_t *labels) {
+    if (!p || !labels || p->dtype != GPR_U32) return GPR_E_INVAL;
+    gpr_graph *g = p->g;
+    const uint32_t n = g->n;
+    uint32_t *minorig = (uint32_t *)g->scan;
+    uint32_t *out = minorig + n;
+    const unsigned grid = grid_for(n, 256, 8192);
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, g->stream, minorig, n, 0xffffffffu);
+    hipLaunchKernelGGL(k_recode_min, dim3(grid), dim3(256), 0, g->stream, (const uint32_t *)p->d, g->order, n, minorig);
+    hipLaunchKernelGGL(k_recode_out, dim3(grid), dim3(256), 0, g->stream, (const uint32_t *)p->d, g->order, n, minorig, out);
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(labels, out, 4ull * n, hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_graph_order(const gpr_graph *g, uint32_t *order) {
+    if (!g || !order) return GPR_E_INVAL;
+    if (!g->order) {
+        for (uint32_t i = 0; i < g->n; ++i) order[i] = i;
+        return GPR_OK;
+    }
+    GPR_HIP(hipMemcpyAsync(order, g->order, 4ull * g->n, hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return GPR_OK;
+}
+
+extern "C" int gpr_graph_remap_id(gpr_graph *g, uint32_t orig, uint32_t *internal) {
+    if (!g || !internal || orig >= g->n) return GPR_E_INVAL;
+    return to_internal_id(g, orig, internal);
+}
+
+extern "C" int gpr_graph_original_id(gpr_graph *g, uint32_t internal, uint32_t *orig) {
+    if (!g || !orig || internal >= g->n) return GPR_E_INVAL;
+    if (!g->order) {
+        *orig = internal;
+        return GPR_OK;
+    }
+    GPR_HIP(hipMemcpyAsync(orig, g->order + internal, 4, hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
     return GPR_OK;
 }
 

@@ -107,7 +107,11 @@ struct gpr_graph {
     uint32_t *long_rows = nullptr;  // nlong ids, ascending
     uint32_t *long_base = nullptr;  // nlong+1 chunk prefix
     uint32_t nlong_chunks = 0;
-    uint32_t hdv = 0;               // highest-degree vertex
+    uint32_t hdv = 0;               // highest-degree vertex (original ID)
+    // vertex relabelling: internal i <-> original order[i]; perm = inverse.
+    // All IDs and per-vertex arrays crossing the C ABI are original.
+    uint32_t *order = nullptr;
+    uint32_t *perm = nullptr;
     hipStream_t stream = nullptr;
     // scratch
     gpr::DevCounters *ctr = nullptr;     // device
@@ -187,6 +191,9 @@ int read_counters(gpr_graph *g);                  // D2H of ctr, synchronises
 int reset_counters(gpr_graph *g);
 int ensure_partials(gpr_graph *g, size_t count);
 int graph_setup(gpr_graph *g);                    // deg, buckets, long rows
+// ID mapping between original (ABI) and internal (relabelled) vertex IDs.
+int map_ids(gpr_graph *g, uint32_t *d_ids, uint32_t k, bool to_internal);
+int to_internal_id(gpr_graph *g, uint32_t orig, uint32_t *internal);
 int deterministic_sum(gpr_graph *g, const double *parts, uint32_t count,
                       double *dst);               // device-side, async
 

@@ -85,6 +85,7 @@ template <> struct Codec<GPR_CONTRIB_CFP16> {
 constexpr int PR_THREADS = 256;
 constexpr int PR_LDS_EDGES = STREAM_ITEMS + (int)LONG_DEG;
 constexpr uint32_t PR_MED = 64;
+constexpr int PR_UNROLL = PR_LDS_EDGES / PR_THREADS;  // 16
 
 template <int FMT>
 __global__ void __launch_bounds__(PR_THREADS)
@@ -118,7 +119,21 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
         const uint64_t cb = rb + (uint64_t)(c - long_base[li]) * LONG_CHUNK;
         const uint64_t ce = min(re, cb + LONG_CHUNK);
         acc_t acc = 0;
-        for (uint64_t e = cb + threadIdx.x; e < ce; e += PR_THREADS) acc += (acc_t)C::dec(contrib[cnbrs[e]]);
+        // PR_UNROLL independent neighbour loads, then as many gathers, in
+        // flight per lane
+        for (uint64_t base = cb; base < ce; base += (uint64_t)PR_THREADS * PR_UNROLL) {
+            uint32_t s[PR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) {
+                const uint64_t e = base + (uint64_t)u * PR_THREADS + threadIdx.x;
+                s[u] = e < ce ? __builtin_nontemporal_load(&cnbrs[e]) : 0xffffffffu;
+            }
+            float v[PR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(contrib[s[u]]) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) acc += (acc_t)v[u];
+        }
         double tot = block_sum((double)acc, dred);
         if (threadIdx.x == 0) {
             long_partial[c] = tot;
@@ -138,8 +153,37 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
         const uint32_t nrows = r1x - r0;
         const uint32_t ne = (uint32_t)(cindex[r1x] - e0);
         if (threadIdx.x == 0) nmed = 0;
-        for (uint32_t i = threadIdx.x; i <= nrows; i += PR_THREADS) rs[i] = (uint32_t)(cindex[r0 + i] - e0);
-        for (uint32_t i = threadIdx.x; i < ne; i += PR_THREADS) vals[i] = C::dec(contrib[cnbrs[e0 + i]]);
+        for (uint32_t base = 0; base <= nrows; base += PR_THREADS * 8) {
+            uint64_t ix[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t i = base + u * PR_THREADS + threadIdx.x;
+                ix[u] = i <= nrows ? __builtin_nontemporal_load(&cindex[r0 + i]) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const uint32_t i = base + u * PR_THREADS + threadIdx.x;
+                if (i <= nrows) rs[i] = (uint32_t)(ix[u] - e0);
+            }
+        }
+        // a bucket holds < PR_LDS_EDGES edges: one round of PR_UNROLL
+        // independent neighbour loads and gathers per lane covers it
+        for (uint32_t base = 0; base < ne; base += PR_THREADS * PR_UNROLL) {
+            uint32_t s[PR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) {
+                const uint32_t i = base + u * PR_THREADS + threadIdx.x;
+                s[u] = i < ne ? __builtin_nontemporal_load(&cnbrs[e0 + i]) : 0xffffffffu;
+            }
+            float v[PR_UNROLL];
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(contrib[s[u]]) : 0.0f;
+#pragma unroll
+            for (int u = 0; u < PR_UNROLL; ++u) {
+                const uint32_t i = base + u * PR_THREADS + threadIdx.x;
+                if (i < ne) vals[i] = v[u];
+            }
+        }
         __syncthreads();
         for (uint32_t i = threadIdx.x; i < nrows; i += PR_THREADS) {
             const uint32_t a = rs[i], z = rs[i + 1];

@@ -70,9 +70,23 @@ int gpr_graph_generate_rmat(uint32_t scale, uint32_t edge_factor, double a,
                             double b, double c, uint64_t seed, gpr_graph **out);
 int gpr_graph_info(const gpr_graph *g, uint32_t *n, uint64_t *m,
                    int *symmetric);
-/* Copy the CSR back (index: n+1 entries, nbrs: m).  Either may be NULL. */
+/* Renumber the vertices internally (GraphVEBOGraptor's VEBO relabelling,
+ * graph/CGraphVEBOGraptor.h:60-142, graph/VEBOReorder.h:134-209).
+ * GPR_ORDER_DEGREE: decreasing degree, ties by ascending ID (symmetric
+ * graphs).  Every vertex ID and per-vertex array that crosses this ABI stays
+ * in original IDs (remapID/originalID, CGraphVEBOGraptor.h:441-500); values
+ * that are themselves vertex IDs (CC labels) are internal until recoded by
+ * gpr_cc_recode. */
+enum { GPR_ORDER_NONE = 0, GPR_ORDER_DEGREE = 1 };
+int gpr_graph_reorder(gpr_graph *g, int mode);
+/* order[i] = original ID of internal vertex i (identity if not reordered). */
+int gpr_graph_order(const gpr_graph *g, uint32_t *order);
+int gpr_graph_remap_id(gpr_graph *g, uint32_t orig, uint32_t *internal);
+int gpr_graph_original_id(gpr_graph *g, uint32_t internal, uint32_t *orig);
+/* Copy the CSR back in internal IDs (index: n+1 entries, nbrs: m).  Either
+ * may be NULL.  With gpr_graph_order this describes the original graph. */
 int gpr_graph_download(const gpr_graph *g, uint64_t *index, uint32_t *nbrs);
-/* Highest out-degree vertex, ties to the lowest ID
+/* Highest out-degree vertex, ties to the lowest ID, as an original ID
  * (GraphCSx::findHighestDegreeVertex, GraphCSx.h:2446-2451). */
 int gpr_graph_highest_degree_vertex(const gpr_graph *g, uint32_t *v);
 int gpr_graph_destroy(gpr_graph *g);
@@ -83,6 +97,9 @@ int gpr_prop_create(gpr_graph *g, int dtype, gpr_prop **out);
 int gpr_prop_upload(gpr_prop *p, const void *host);
 int gpr_prop_download(const gpr_prop *p, void *host);
 int gpr_prop_destroy(gpr_prop *p);
+/* CCv recode (bench/CCv.C:121-169): from a u32 label property in internal
+ * IDs, labels[v] = minimum original vertex ID of v's component. */
+int gpr_cc_recode(const gpr_prop *p, uint32_t *labels);
 
 /* ---- frontiers: frontier (frontier/frontier.h:58-1179) ------------------ */
 enum {

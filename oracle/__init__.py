@@ -43,8 +43,8 @@ def lib():
         L = C.CDLL(_SO)
         L.orc_mix64.restype = C.c_uint64
         L.orc_mix64.argtypes = [C.c_uint64]
-        L.orc_rmat_thresholds.argtypes = [C.c_double, C.c_double, C.c_double, u32p]
-        L.orc_rmat_edges.argtypes = [C.c_uint32, u32p, C.c_uint64, C.c_uint64, C.c_uint64, u32p, u32p]
+        L.orc_rmat_thresholds.argtypes = [C.c_double, C.c_double, C.c_double, u64p]
+        L.orc_rmat_edges.argtypes = [C.c_uint32, u64p, C.c_uint64, C.c_uint64, C.c_uint64, u32p, u32p]
         L.orc_er_edges.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p]
         L.orc_build_csr.restype = C.c_uint64
         L.orc_build_csr.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, C.c_int, u64p, u32p]
@@ -84,7 +84,7 @@ DEFAULT_SEED = 0x47524150
 
 
 def rmat_thresholds(a, b, c):
-    t = np.zeros(3, np.uint32)
+    t = np.zeros(3, np.uint64)
     lib().orc_rmat_thresholds(a, b, c, t)
     return t
 

@@ -30,16 +30,18 @@ uint64_t orc_mix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-void orc_rmat_thresholds(double a, double b, double c, uint32_t t[3]) {
-    t[0] = (uint32_t)(a * 4294967296.0);
-    t[1] = (uint32_t)((a + b) * 4294967296.0);
-    t[2] = (uint32_t)((a + b + c) * 4294967296.0);
+/* Thresholds are 64-bit so that a probability of 1 maps to 2^32 and
+ * every 32-bit draw falls below it. */
+void orc_rmat_thresholds(double a, double b, double c, uint64_t t[3]) {
+    t[0] = (uint64_t)(a * 4294967296.0);
+    t[1] = (uint64_t)((a + b) * 4294967296.0);
+    t[2] = (uint64_t)((a + b + c) * 4294967296.0);
 }
 
 /* RMAT: for each of `scale` levels (most significant bit first) pick a
  * quadrant with probabilities a (0,0), b (0,1), c (1,0), d (1,1).  Level l
  * uses half (l & 1) of word h(e, l/2). */
-void orc_rmat_edges(uint32_t scale, const uint32_t t[3], uint64_t seed,
+void orc_rmat_edges(uint32_t scale, const uint64_t t[3], uint64_t seed,
                     uint64_t first, uint64_t count,
                     uint32_t *src, uint32_t *dst) {
 #pragma omp parallel for schedule(static)

@@ -28,10 +28,10 @@ extern "C" {
 
 /* ---- input generation (no generator exists in the reference) ---------- */
 uint64_t orc_mix64(uint64_t x);
-/* Quadrant thresholds as unsigned 32-bit fractions of 2^32. */
-void orc_rmat_thresholds(double a, double b, double c, uint32_t t[3]);
+/* Quadrant thresholds floor(p * 2^32) for p = a, a+b, a+b+c (64-bit). */
+void orc_rmat_thresholds(double a, double b, double c, uint64_t t[3]);
 /* Edges [first, first+count) of an RMAT(scale) stream with given seed. */
-void orc_rmat_edges(uint32_t scale, const uint32_t t[3], uint64_t seed,
+void orc_rmat_edges(uint32_t scale, const uint64_t t[3], uint64_t seed,
                     uint64_t first, uint64_t count,
                     uint32_t *src, uint32_t *dst);
 /* Erdos-Renyi style stream: count uniform endpoint pairs in [0,n). */

@@ -1,0 +1,350 @@
+"""GPU parity tests: the HIP engine against the CPU oracle, through the C ABI.
+
+Bars (stated per test):
+  BFS levels, CC labels, generator output, frontiers: bit-exact.
+  PageRank FP32 (FP32 store, FP64 accumulate): relative L1 <= 1e-6 vs the
+  FP64 oracle.  16-bit contributions: relative L1 <= 1e-3 (bf16, fp16,
+  cfp16) and <= 1e-2 (bf16 truncating), each run against the same FP64
+  oracle; sum of ranks 1 within 1e-6.
+"""
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+G = pytest.importorskip("graptor_amd")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    G.lib()   # fails loudly when the library or the gfx950 device is missing
+    yield
+
+
+def edges_of(index):
+    return np.repeat(np.arange(len(index) - 1), np.diff(index).astype(np.int64))
+
+
+def named_graphs():
+    out = {}
+    n = 300
+    s = np.arange(n - 1, dtype=np.uint32)
+    out["path"] = O.build_csr(n, s, s + 1)
+    out["star"] = O.build_csr(2000, np.zeros(1999, np.uint32), np.arange(1, 2000, dtype=np.uint32))
+    out["grid"] = O.grid_graph(40, 30, 0.01, seed=3)
+    out["er"] = O.er_graph(5000, 6, seed=11)
+    out["rmat10"] = O.rmat_graph(10)
+    out["rmat14"] = O.rmat_graph(14)
+    out["rmat16_ef8_a045"] = O.rmat_graph(16, 8, (0.45, 0.25, 0.25))
+    out["single"] = (np.zeros(2, np.uint64), np.zeros(0, np.uint32))
+    out["two_isolated_pairs"] = O.build_csr(6, np.array([0, 3], np.uint32), np.array([1, 4], np.uint32))
+    return out
+
+
+GRAPHS = named_graphs()
+DIRS = [G.DIR_AUTO, G.DIR_PUSH, G.DIR_PULL]
+KINDS = [0, G.FRONTIER_LIST, G.FRONTIER_BITMAP]
+
+
+@pytest.fixture(scope="module", params=sorted(GRAPHS), ids=sorted(GRAPHS))
+def graph(request):
+    index, nbrs = GRAPHS[request.param]
+    g = G.Graph.from_csr(index, nbrs)
+    yield request.param, index, nbrs, g
+    g.destroy()
+
+
+@pytest.fixture(scope="module", params=sorted(GRAPHS), ids=sorted(GRAPHS))
+def rgraph(request):
+    """The same graphs, relabelled by degree inside the engine."""
+    index, nbrs = GRAPHS[request.param]
+    g = G.Graph.from_csr(index, nbrs).reorder()
+    yield request.param, index, nbrs, g
+    g.destroy()
+
+
+def sources(index):
+    n = len(index) - 1
+    return sorted({0, n - 1, O.highest_degree_vertex(index), n // 2})
+
+
+@pytest.mark.parametrize("d", DIRS)
+@pytest.mark.parametrize("k", KINDS)
+def test_bfs_levels_bit_exact(graph, d, k):
+    name, index, nbrs, g = graph
+    for s in sources(index):
+        ref = O.bfs_levels(index, nbrs, s)
+        lv, info = g.bfslvl(s, direction=d, out_kind=k)
+        assert np.array_equal(lv, ref), (name, s, d, k)
+        n = len(index) - 1
+        assert info.edges == int(np.diff(index)[ref <= n].sum())
+
+
+@pytest.mark.parametrize("d", DIRS)
+def test_bfs_reordered_bit_exact(rgraph, d):
+    name, index, nbrs, g = rgraph
+    for s in sources(index):
+        lv, _ = g.bfslvl(s, direction=d)
+        assert np.array_equal(lv, O.bfs_levels(index, nbrs, s)), (name, s, d)
+
+
+@pytest.mark.parametrize("d", DIRS)
+@pytest.mark.parametrize("k", KINDS)
+def test_cc_labels_bit_exact(graph, d, k):
+    name, index, nbrs, g = graph
+    lab, _ = g.ccv(direction=d, out_kind=k)
+    assert np.array_equal(lab, O.cc_labels(index, nbrs)), (name, d, k)
+
+
+@pytest.mark.parametrize("d", DIRS)
+def test_cc_reordered_bit_exact(rgraph, d):
+    name, index, nbrs, g = rgraph
+    lab, _ = g.ccv(direction=d)
+    assert np.array_equal(lab, O.cc_labels(index, nbrs)), (name, d)
+
+
+TOL = {"fp32": 1e-6, "bf16": 1e-3, "fp16": 1e-3, "cfp16": 1e-3, "bf16t": 1e-2}
+
+
+@pytest.mark.parametrize("fmt", sorted(TOL))
+def test_pagerank_vs_fp64(graph, fmt):
+    name, index, nbrs, g = graph
+    if len(nbrs) == 0:
+        pytest.skip("no edges")
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    x, info = g.pr_vary([(fmt, 0.0)], max_iter=20)
+    err = np.abs(x - x64).sum() / np.abs(x64).sum()
+    assert err <= TOL[fmt], (name, fmt, err)
+    assert abs(float(x.sum()) - 1.0) < 1e-6
+    assert info.iters == 20
+
+
+def test_pagerank_reordered_and_deterministic(rgraph):
+    name, index, nbrs, g = rgraph
+    if len(nbrs) == 0:
+        pytest.skip("no edges")
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    x1, _ = g.prv(max_iter=20)
+    x2, _ = g.prv(max_iter=20)
+    assert np.array_equal(x1, x2)                 # deterministic reductions
+    assert np.abs(x1 - x64).sum() / x64.sum() <= 1e-6
+
+
+def test_pagerank_tolerance_and_stages():
+    index, nbrs = GRAPHS["rmat14"]
+    with G.Graph.from_csr(index, nbrs) as g:
+        x64, it64, _ = O.pr_fp64(index, nbrs, 1000, 1e-7)
+        x, info = g.prv(max_iter=1000, tol=1e-7)
+        assert info.delta <= 1e-7 and abs(info.iters - it64) <= 2
+        assert np.abs(x - x64).sum() < 1e-6
+        # PR_vary staging (PR_vary.C:652-757): 16-bit to 1e-3, FP32 to 1e-7
+        xs, infos = g.pr_vary([("cfp16", 1e-3), ("fp32", 1e-7)], max_iter=1000)
+        assert infos.delta <= 1e-7
+        assert np.abs(xs - x64).sum() < 2e-6
+
+
+def test_pagerank_closed_form_cycle():
+    n = 1000
+    s = np.arange(n, dtype=np.uint32)
+    index, nbrs = O.build_csr(n, s, (s + 1) % n)
+    with G.Graph.from_csr(index, nbrs) as g:
+        x, _ = g.prv(max_iter=30)
+        assert np.abs(x - 1.0 / n).max() < 1e-9
+
+
+def test_generator_bit_exact():
+    for scale, ef, abc in ((10, 16, O.RMAT_DEFAULT), (16, 16, O.RMAT_DEFAULT), (12, 4, (1.0, 0.0, 0.0)),
+                           (12, 8, (0.45, 0.2, 0.2))):
+        with G.Graph.rmat(scale, ef, *abc) as g:
+            i1, n1 = g.download()
+        i2, n2 = O.rmat_graph(scale, ef, abc)
+        assert np.array_equal(i1, i2) and np.array_equal(n1, n2), (scale, ef, abc)
+
+
+def test_reorder_is_degree_sorted_relabelling():
+    index, nbrs = GRAPHS["rmat14"]
+    with G.Graph.from_csr(index, nbrs).reorder() as g:
+        ii, nn = g.download()
+        order = g.order()
+        assert np.array_equal(np.sort(order), np.arange(len(index) - 1))
+        deg = np.diff(index)
+        assert np.array_equal(np.diff(ii), deg[order])
+        assert (np.diff(np.diff(ii).astype(np.int64)) <= 0).all()
+        perm = np.empty_like(order)
+        perm[order] = np.arange(len(order), dtype=np.uint32)
+        # internal row i lists perm[] of the original row order[i], sorted
+        for i in (0, 1, 5, len(order) // 2):
+            row = np.sort(perm[nbrs[index[order[i]]:index[order[i] + 1]]])
+            assert np.array_equal(nn[ii[i]:ii[i + 1]], row)
+        assert g.original_id(0) == O.highest_degree_vertex(index)
+        assert g.remap_id(g.original_id(7)) == 7
+
+
+def test_b2_load_save(tmp_path):
+    index, nbrs = GRAPHS["rmat10"]
+    p = str(tmp_path / "g.b2")
+    O.b2_write(p, index, nbrs)
+    with G.Graph.load_b2(p) as g:
+        i2, n2 = g.download()
+        assert np.array_equal(i2, index) and np.array_equal(n2, nbrs)
+        q = str(tmp_path / "h.b2")
+        g.save_b2(q)
+    assert open(p, "rb").read() == open(q, "rb").read()
+    bad = str(tmp_path / "bad.b2")
+    with open(bad, "wb") as f:
+        f.write(struct.pack("<8Q", 2, 1, 10, 5, 4, 8, 0, 0))   # truncated body
+    with pytest.raises(G.GprError) as e:
+        G.Graph.load_b2(bad)
+    assert e.value.code == G.GPR_E_FORMAT
+
+
+def test_invalid_inputs_rejected():
+    index = np.array([0, 1, 2], np.uint64)
+    with pytest.raises(G.GprError) as e:
+        G.Graph.from_csr(index, np.array([1, 5], np.uint32))
+    assert e.value.code == G.GPR_E_FORMAT
+    with G.Graph.from_csr(*GRAPHS["path"]) as g:
+        with pytest.raises(G.GprError):
+            g.bfslvl(10 ** 6)
+        f = g.frontier()
+        with pytest.raises(G.GprError):
+            f.set_list([0, 10 ** 6])
+        f.destroy()
+
+
+def test_non_symmetric_graph_pull_uses_csc():
+    rng = np.random.default_rng(4)
+    n = 3000
+    src = rng.integers(0, n, 20000).astype(np.uint32)
+    dst = rng.integers(0, n, 20000).astype(np.uint32)
+    index, nbrs = O.build_csr(n, src, dst, symmetrize=False)
+    with G.Graph.from_csr(index, nbrs, symmetric=False) as g:
+        for d in DIRS:
+            lv, _ = g.bfslvl(5, direction=d)
+            assert np.array_equal(lv, O.bfs_levels(index, nbrs, 5)), d
+        # PR on a directed graph: CSC pulls in-edges, out-degrees divide
+        ci, cn = O.build_csr(n, dst, src, symmetrize=False)
+        x64, _, _ = O.pr_fp64(ci, cn, 20, 0.0, outdeg=np.diff(index).astype(np.uint32))
+        x, _ = g.prv(max_iter=20)
+        # dangling vertices leak mass and are redistributed: compare shape
+        assert np.abs(x - x64).sum() / x64.sum() <= 1e-6
+
+
+# ---- engine primitives ---------------------------------------------------------
+def test_frontier_conversions_and_stats():
+    index, nbrs = GRAPHS["rmat14"]
+    deg = np.diff(index)
+    rng = np.random.default_rng(1)
+    vs = np.unique(rng.integers(0, len(index) - 1, 3000)).astype(np.uint32)
+    for reorder in (False, True):
+        g = G.Graph.from_csr(index, nbrs)
+        if reorder:
+            g.reorder()
+        f = g.frontier().set_list(vs[::-1].copy())
+        nv, ne, kind = f.stats()
+        assert (nv, ne, kind) == (len(vs), int(deg[vs].sum()), G.FRONTIER_LIST)
+        f.convert(G.FRONTIER_BITMAP)
+        assert np.array_equal(f.download(), vs)
+        nv, ne, kind = f.stats()
+        assert (nv, ne) == (len(vs), int(deg[vs].sum()))
+        f.destroy()
+        f = g.frontier().set_neighbours(int(vs[0]))
+        assert np.array_equal(f.download(), nbrs[index[vs[0]]:index[vs[0] + 1]])
+        f.destroy()
+        g.destroy()
+
+
+def test_single_edgemap_step_matches_rule():
+    """One BFS step from a random frontier, push and pull, equals the
+    relax/filter/record rule evaluated on the CPU."""
+    index, nbrs = GRAPHS["rmat14"]
+    n = len(index) - 1
+    rng = np.random.default_rng(9)
+    level = np.full(n, n + 1, np.uint32)
+    F = np.unique(rng.integers(0, n, 500)).astype(np.uint32)
+    level[F] = 3
+    visited = np.unique(rng.integers(0, n, 2000)).astype(np.uint32)
+    visited = np.setdiff1d(visited, F)
+    level[visited] = 2
+    # expected: d with level > 4 and a neighbour in F gets 4
+    exp = level.copy()
+    src = edges_of(index)
+    inF = np.zeros(n, bool)
+    inF[F] = True
+    hit = np.zeros(n, bool)
+    hit[nbrs[inF[src]]] = True
+    upd = hit & (level > 4)
+    exp[upd] = 4
+    for d in (G.DIR_PUSH, G.DIR_PULL):
+        for k in (G.FRONTIER_LIST, G.FRONTIER_BITMAP):
+            with G.Graph.from_csr(index, nbrs) as g:
+                p = g.prop(G.U32)
+                p.upload(level)
+                fin = g.frontier().set_list(F)
+                fout = g.frontier()
+                st = g.edgemap(G.RELAX_MIN_PLUS1, p, fin, fout, dst_filter=G.DST_GT, dst_arg=4, direction=d,
+                               out_kind=k)
+                assert np.array_equal(p.download(), exp), (d, k)
+                assert np.array_equal(fout.download(), np.flatnonzero(upd).astype(np.uint32)), (d, k)
+                assert st["nactv_out"] == upd.sum() and st["nacte_out"] == int(np.diff(index)[upd].sum())
+                fin.destroy()
+                fout.destroy()
+                p.destroy()
+
+
+def test_vertex_filter_and_maps():
+    index, nbrs = GRAPHS["er"]
+    n = len(index) - 1
+    with G.Graph.from_csr(index, nbrs) as g:
+        p = g.prop(G.U32)
+        p.vertex_map(G.VMAP_IOTA)
+        assert np.array_equal(p.download(), np.arange(n, dtype=np.uint32))
+        p.vertex_map(G.VMAP_FILL, 7)
+        assert (p.download() == 7).all()
+        f = g.frontier().set_list([1, 5, 9])
+        p.vertex_map(G.VMAP_FILL_FRONTIER, 0, frontier=f)
+        got = p.download()
+        assert (got[[1, 5, 9]] == 0).all() and (np.delete(got, [1, 5, 9]) == 7).all()
+        f.destroy()
+        p.destroy()
+
+
+# ---- full-size properties (configs: RMAT-20 BFS; RMAT-24 CC and PR) -----------
+@pytest.mark.parametrize("scale", [20])
+def test_rmat20_bfs_full_size(scale):
+    with G.Graph.rmat(scale) as g:
+        index, nbrs = g.download()
+        s = g.highest_degree_vertex()
+        g.reorder()
+        lv, info = g.bfslvl(s)
+        assert np.array_equal(lv, O.bfs_levels(index, nbrs, s))
+        lab, _ = g.ccv()
+        assert np.array_equal(lab, O.cc_labels(index, nbrs))
+
+
+def test_rmat24_invariants():
+    """RMAT-24 (16.7M vertices, ~500M edges): size-independent properties
+    (BFSLVLv.C:282-296 level invariant, CCv.C:605-660 neighbour check, PR
+    sum and agreement with the OpenMP restatement after 3 iterations)."""
+    with G.Graph.rmat(24) as g:
+        s = g.highest_degree_vertex()
+        g.reorder()
+        index, nbrs = g.download()
+        order = g.order()
+        lv, _ = g.bfslvl(s)
+        lab, _ = g.ccv()
+        li = lv[order].astype(np.int64)
+        src = edges_of(index)
+        assert (np.abs(li[src] - li[nbrs]) <= 1).all()
+        ci = lab[order]
+        assert (ci[src] == ci[nbrs]).all()
+        assert (lab <= np.arange(len(lab))).all()
+        x, info = g.prv(max_iter=3)
+        assert abs(float(np.sum(x, dtype=np.float64)) - 1.0) < 1e-5
+        xc, _, _ = O.graptor_pr(index, nbrs, 3, 0.0)
+        assert np.abs(x[order] - xc).sum() / xc.sum() < 1e-6

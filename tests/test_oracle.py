@@ -1,0 +1,331 @@
+"""CPU tests that pin the oracle (oracle/oracle.c) before it is trusted.
+
+The reference ships no golden vectors for this path (its only tests are x86
+SIMD trait tests; the bench ctest is disabled and its graphs are absent), and
+it cannot be compiled here (C++23 <stdfloat> and libconfig are missing), so
+each oracle function is pinned by mathematics instead:
+
+  orc_mix64            splitmix64 published known answer (seed 0 ->
+                       0xe220a8397b1dcdaf) and the avalanche of one input bit
+  orc_rmat_*           degenerate probabilities (a=1: all self loops; b=1:
+                       edge (0, n-1)), quadrant frequencies vs a/b/c/d
+  orc_build_csr        brute-force set construction on random edge lists
+  orc_b2_*             byte layout of the header, v2/v3 reads, bad headers
+  orc_bfs_levels       brute-force Bellman-Ford hop distances; path closed form
+  orc_cc_labels        brute-force DFS components; min-ID labels
+  orc_pr_fp64          dense-matrix power iteration; cycle -> uniform 1/n;
+                       complete bipartite closed form
+  orc_graptor_*        equal to the textbook oracles (BFS/CC bit-exact, PR
+                       within FP32 rounding)
+  orc_cfp16_*          exhaustive 2^16 decode against the closed form
+                       (1 + m/1024) * 2^(e-63), truncation of every encode
+  orc_bf16*            truncation vs the upper half; RNE vs torch.bfloat16
+"""
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def brute_csr(n, src, dst, sym=True):
+    s = set()
+    for u, v in zip(src.tolist(), dst.tolist()):
+        if u == v:
+            continue
+        s.add((u, v))
+        if sym:
+            s.add((v, u))
+    e = sorted(s)
+    index = np.zeros(n + 1, np.uint64)
+    for u, _ in e:
+        index[u + 1] += 1
+    index = np.cumsum(index).astype(np.uint64)
+    return index, np.array([v for _, v in e], np.uint32)
+
+
+def bf_levels(index, nbrs, s):
+    n = len(index) - 1
+    INF = n + 1
+    lv = [INF] * n
+    lv[s] = 0
+    changed = True
+    while changed:
+        changed = False
+        for u in range(n):
+            if lv[u] == INF:
+                continue
+            for e in range(int(index[u]), int(index[u + 1])):
+                v = int(nbrs[e])
+                if lv[u] + 1 < lv[v]:
+                    lv[v] = lv[u] + 1
+                    changed = True
+    return np.array(lv, np.uint32)
+
+
+def dfs_labels(index, nbrs):
+    n = len(index) - 1
+    lab = [-1] * n
+    for s in range(n):
+        if lab[s] >= 0:
+            continue
+        stack, comp = [s], []
+        lab[s] = s
+        while stack:
+            u = stack.pop()
+            comp.append(u)
+            for e in range(int(index[u]), int(index[u + 1])):
+                v = int(nbrs[e])
+                if lab[v] < 0:
+                    lab[v] = s
+                    stack.append(v)
+        m = min(comp)
+        for u in comp:
+            lab[u] = m
+    return np.array(lab, np.uint32)
+
+
+def small_graphs():
+    rng = np.random.default_rng(7)
+    out = []
+    for n, k in ((1, 0), (2, 1), (7, 9), (30, 40), (64, 300), (200, 150)):
+        src = rng.integers(0, n, k).astype(np.uint32)
+        dst = rng.integers(0, n, k).astype(np.uint32)
+        out.append((n, src, dst))
+    return out
+
+
+def test_mix64_known_answer():
+    assert O.lib().orc_mix64(0) == 0xE220A8397B1DCDAF
+    # avalanche: one flipped input bit flips about half the output bits
+    flips = [bin(O.lib().orc_mix64(1 << i) ^ O.lib().orc_mix64(0)).count("1") for i in range(64)]
+    assert 20 < np.mean(flips) < 44
+
+
+def test_rmat_thresholds():
+    t = O.rmat_thresholds(0.5, 0.25, 0.125)
+    assert list(t) == [1 << 31, 3 << 30, 7 << 29]
+
+
+def test_rmat_degenerate():
+    src, dst = O.rmat_edges(8, 4, (1.0, 0.0, 0.0))
+    assert (src == 0).all() and (dst == 0).all()
+    index, nbrs = O.build_csr(256, src, dst)
+    assert len(nbrs) == 0 and (index == 0).all()
+    src, dst = O.rmat_edges(8, 4, (0.0, 1.0, 0.0))
+    assert (src == 0).all() and (dst == 255).all()
+    index, nbrs = O.build_csr(256, src, dst)
+    assert list(nbrs) == [255, 0]
+
+
+def test_rmat_quadrant_frequencies():
+    a, b, c = 0.57, 0.19, 0.19
+    src, dst = O.rmat_edges(1, 100000, (a, b, c))
+    q = src * 2 + dst
+    freq = np.bincount(q, minlength=4) / len(q)
+    for f, p in zip(freq, (a, b, c, 1 - a - b - c)):
+        assert abs(f - p) < 4 * np.sqrt(p * (1 - p) / len(q))
+
+
+def test_rmat_stream_is_counter_based():
+    s1, d1 = O.rmat_edges(12, 16)
+    s2, d2 = O.rmat_edges(12, 16, first=1000, count=500)
+    assert (s1[1000:1500] == s2).all() and (d1[1000:1500] == d2).all()
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_build_csr_brute(case):
+    n, src, dst = small_graphs()[case]
+    for sym in (True, False):
+        i1, n1 = O.build_csr(n, src, dst, sym)
+        i2, n2 = brute_csr(n, src, dst, sym)
+        assert np.array_equal(i1, i2) and np.array_equal(n1, n2)
+        assert O.check_csr(i1, n1) == 0
+
+
+def test_check_csr_detects_errors():
+    index = np.array([0, 2, 3], np.uint64)
+    assert O.check_csr(index, np.array([1, 0, 0], np.uint32)) == 4   # unsorted row 0
+    assert O.check_csr(index, np.array([0, 0, 0], np.uint32)) == 4   # duplicate
+    assert O.check_csr(index, np.array([0, 5, 0], np.uint32)) == 3   # out of range
+    assert O.check_csr(np.array([0, 2, 4], np.uint64), np.array([0, 1, 0], np.uint32)) == 1  # index[n] != m
+
+
+def test_b2_layout_and_versions(tmp_path):
+    index, nbrs = O.rmat_graph(8, 4)
+    p = str(tmp_path / "g.b2")
+    O.b2_write(p, index, nbrs)
+    raw = open(p, "rb").read()
+    n, m = len(index) - 1, len(nbrs)
+    assert struct.unpack("<8Q", raw[:64]) == (2, 1, n, m, 4, 8, 0, 0)
+    assert len(raw) == 64 + 8 * n + 4 * m   # index[n] is not stored (GraphCSx.h:2023-2024)
+    i2, n2 = O.b2_read(p)
+    assert np.array_equal(index, i2) and np.array_equal(nbrs, n2)
+    # version 3 stores n+1 index entries (GraphCSx.h:2255-2256)
+    p3 = str(tmp_path / "g3.b2")
+    with open(p3, "wb") as f:
+        f.write(struct.pack("<8Q", 3, 1, n, m, 4, 8, 0, 0))
+        f.write(index.tobytes())
+        f.write(nbrs.tobytes())
+    i3, n3 = O.b2_read(p3)
+    assert np.array_equal(index, i3) and np.array_equal(nbrs, n3)
+    bad = str(tmp_path / "bad.b2")
+    with open(bad, "wb") as f:
+        f.write(struct.pack("<8Q", 4, 1, n, m, 4, 8, 0, 0))
+    with pytest.raises(IOError):
+        O.b2_read(bad)
+
+
+def path_graph(n):
+    src = np.arange(n - 1, dtype=np.uint32)
+    return O.build_csr(n, src, src + 1)
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_bfs_brute(case):
+    n, src, dst = small_graphs()[case]
+    index, nbrs = O.build_csr(n, src, dst)
+    for s in range(0, n, max(1, n // 5)):
+        ref = bf_levels(index, nbrs, s)
+        assert np.array_equal(O.bfs_levels(index, nbrs, s), ref)
+        lv, _ = O.graptor_bfs(index, nbrs, s, 4)
+        assert np.array_equal(lv, ref)
+
+
+def test_bfs_path_closed_form():
+    index, nbrs = path_graph(100)
+    lv = O.bfs_levels(index, nbrs, 37)
+    assert np.array_equal(lv, np.abs(np.arange(100) - 37).astype(np.uint32))
+
+
+def test_bfs_rmat_invariant_and_graptor():
+    index, nbrs = O.rmat_graph(14)
+    s = O.highest_degree_vertex(index)
+    lv = O.bfs_levels(index, nbrs, s)
+    n = len(index) - 1
+    src = np.repeat(np.arange(n), np.diff(index).astype(np.int64))
+    a, b = lv[src].astype(np.int64), lv[nbrs].astype(np.int64)
+    # BFSLVLv.C:282-296 symmetric-graph validation
+    assert (np.abs(a - b) <= 1).all()
+    isolated = np.diff(index) == 0
+    assert (lv[isolated & (np.arange(n) != s)] == n + 1).all()
+    lv2, info = O.graptor_bfs(index, nbrs, s, 8)
+    assert np.array_equal(lv, lv2)
+    assert info.push_iters >= 1 and info.iters > info.push_iters   # both directions used
+
+
+@pytest.mark.parametrize("case", range(6))
+def test_cc_brute(case):
+    n, src, dst = small_graphs()[case]
+    index, nbrs = O.build_csr(n, src, dst)
+    ref = dfs_labels(index, nbrs)
+    assert np.array_equal(O.cc_labels(index, nbrs), ref)
+    lab, _ = O.graptor_cc(index, nbrs, 4)
+    assert np.array_equal(lab, ref)
+
+
+def test_cc_rmat_graptor():
+    index, nbrs = O.rmat_graph(15)
+    ref = O.cc_labels(index, nbrs)
+    n = len(index) - 1
+    src = np.repeat(np.arange(n), np.diff(index).astype(np.int64))
+    assert (ref[src] == ref[nbrs]).all()           # CCv.C:605-660 neighbour check
+    assert (ref <= np.arange(n)).all()             # label is the minimum ID
+    lab, _ = O.graptor_cc(index, nbrs, 8)
+    assert np.array_equal(lab, ref)
+
+
+def test_pr_dense_matrix():
+    for n, src, dst in small_graphs()[2:]:
+        index, nbrs = O.build_csr(n, src, dst)
+        x, it, _ = O.pr_fp64(index, nbrs, 15, 0.0)
+        xd = O.pr_dense_fp64(index, nbrs, 15)
+        assert it == 15
+        assert np.abs(x - xd).max() < 1e-12
+
+
+def test_pr_closed_forms():
+    n = 50
+    src = np.arange(n, dtype=np.uint32)
+    index, nbrs = O.build_csr(n, src, (src + 1) % n)      # cycle: 2-regular
+    x, _, _ = O.pr_fp64(index, nbrs, 30, 0.0)
+    assert np.abs(x - 1.0 / n).max() < 1e-15
+    # complete bipartite K(p,q): ranks depend only on the side; the fixed
+    # point of x_p = 0.15/n + 0.85*q*x_q/p ... solves to x_p/x_q below
+    p, q = 3, 5
+    s, d = np.meshgrid(np.arange(p), np.arange(p, p + q))
+    index, nbrs = O.build_csr(p + q, s.ravel().astype(np.uint32), d.ravel().astype(np.uint32))
+    x, _, _ = O.pr_fp64(index, nbrs, 400, 0.0)
+    n = p + q
+    # x_p = (1-d)/n + d*q*x_q/q... each p-vertex receives from all q-vertices
+    # (outdeg p):  x_p = (1-d)/n + d*q*x_q/p,  x_q = (1-d)/n + d*p*x_p/q
+    A = np.array([[1, -0.85 * q / p], [-0.85 * p / q, 1]])
+    xp, xq = np.linalg.solve(A, [0.15 / n, 0.15 / n])
+    assert abs(x[0] - xp) < 1e-12 and abs(x[-1] - xq) < 1e-12
+    assert abs(x.sum() - 1) < 1e-12
+
+
+def test_pr_rmat_graptor_and_sum():
+    index, nbrs = O.rmat_graph(14)
+    x, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    assert abs(x.sum() - 1) < 1e-12
+    xf, it, _ = O.graptor_pr(index, nbrs, 20, 0.0, 8)
+    assert it == 20
+    assert np.abs(xf - x).sum() / x.sum() < 1e-6
+
+
+def test_pr_tolerance_stops():
+    index, nbrs = O.rmat_graph(12)
+    x, it, d = O.pr_fp64(index, nbrs, 1000, 1e-7)
+    assert d <= 1e-7 and it < 1000
+
+
+def test_cfp16_exhaustive_closed_form():
+    L = O.lib()
+    for h in range(1 << 16):
+        e6, mant = h >> 10, h & 1023
+        v = L.orc_cfp16_decode(h)
+        assert v == np.ldexp(1.0 + mant / 1024.0, e6 - 63)
+        assert L.orc_cfp16_encode(v) == h
+    # truncation: every float in [2^-63, 2) encodes to the code just below it
+    rng = np.random.default_rng(3)
+    xs = np.ldexp(rng.random(20000) + 1.0, rng.integers(-63, 1, 20000)).astype(np.float32)
+    for x in xs:
+        h = L.orc_cfp16_encode(float(x))
+        assert L.orc_cfp16_decode(h) <= x < (L.orc_cfp16_decode(h + 1) if h < 65535 else 2.0)
+
+
+def test_bf16_truncation_and_rne():
+    rng = np.random.default_rng(5)
+    xs = (rng.standard_normal(4000) * 10.0 ** rng.integers(-30, 30, 4000)).astype(np.float32)
+    bits = xs.view(np.uint32)
+    for x, b in zip(xs, bits):
+        assert O.bf16t_encode(float(x)) == int(b) >> 16
+        assert O.bf16_decode(int(b) >> 16) == np.float32(np.uint32(int(b) & 0xFFFF0000).view(np.float32))
+    torch = pytest.importorskip("torch")
+    ref = torch.from_numpy(xs).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    got = np.array([O.bf16_rne_encode(float(x)) for x in xs], np.uint16)
+    assert np.array_equal(got, ref)
+    # ties go to even
+    assert O.bf16_rne_encode(float(np.uint32(0x3F808000).view(np.float32))) == 0x3F80
+    assert O.bf16_rne_encode(float(np.uint32(0x3F818000).view(np.float32))) == 0x3F82
+
+
+def test_planted_mistakes_are_caught():
+    """A dropped symmetrisation or an off-by-one level must fail the pins."""
+    index, nbrs = O.rmat_graph(10)
+    s = O.highest_degree_vertex(index)
+    lv = O.bfs_levels(index, nbrs, s)
+    wrong = lv.copy()
+    wrong[wrong == 2] = 3
+    n = len(index) - 1
+    src = np.repeat(np.arange(n), np.diff(index).astype(np.int64))
+    assert not (np.abs(wrong[src].astype(np.int64) - wrong[nbrs].astype(np.int64)) <= 1).all() or \
+        not np.array_equal(wrong, bf_levels(index, nbrs, s))
+    src_e, dst_e = O.rmat_edges(8, 4)
+    i1, n1 = O.build_csr(256, src_e, dst_e, False)
+    i2, n2 = brute_csr(256, src_e, dst_e, True)
+    assert not (np.array_equal(i1, i2) and np.array_equal(n1, n2))
