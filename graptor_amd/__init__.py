@@ -112,6 +112,13 @@ SIGNATURES = {
     "gpr_ccv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_prv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_pr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_msbfs": (C.c_int, [_P, _P, C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_batch_generate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _PP]),
+    "gpr_batch_create": (C.c_int, [_P, _P, _P, C.c_uint32, _PP]),
+    "gpr_batch_info": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "gpr_batch_download": (C.c_int, [_P, _P, _P, _P]),
+    "gpr_batch_cc": (C.c_int, [_P, _P, C.POINTER(RunInfo)]),
+    "gpr_batch_destroy": (C.c_int, [_P]),
     "gpr_trace_get": (C.c_int, [_P, C.POINTER(IterStats), C.c_int, C.POINTER(C.c_int)]),
     "gpr_profile_enable": (C.c_int, [_P, C.c_int]),
     "gpr_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -274,6 +281,15 @@ class Graph:
         _check(lib().gpr_pr_vary(self.h, arr, len(stages), C.byref(o), _ptr(out), C.byref(info)))
         return out, info
 
+    def msbfs(self, sources, **kw):
+        """Levels from up to 64 sources in one traversal, shape (k, n)."""
+        src = np.ascontiguousarray(sources, np.uint32)
+        out = np.empty((len(src), self.n), np.uint32)
+        o = opts(**kw)
+        info = RunInfo()
+        _check(lib().gpr_msbfs(self.h, _ptr(src), len(src), C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
     def trace(self):
         k = C.c_int()
         _check(lib().gpr_trace_get(self.h, None, 0, C.byref(k)))
@@ -309,6 +325,56 @@ class Graph:
         _check(lib().gpr_edgemap(self.h, C.byref(d), frontier_in.h if frontier_in else None,
                                  frontier_out.h if frontier_out else None, C.byref(st)))
         return st.as_dict()
+
+
+class Batch:
+    """Thousands of small graphs as one CSR with vertex offsets."""
+
+    def __init__(self, handle):
+        self.h = handle
+        c, n, m = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().gpr_batch_info(self.h, C.byref(c), C.byref(n), C.byref(m)))
+        self.count, self.n, self.m = c.value, n.value, m.value
+
+    @classmethod
+    def generate(cls, count=4096, nmin=1024, nmax=65536, seed=0x47524150):
+        h = C.c_void_p()
+        _check(lib().gpr_batch_generate(count, nmin, nmax, seed, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_csr(cls, index, nbrs, voff):
+        index = np.ascontiguousarray(index, np.uint64)
+        nbrs = np.ascontiguousarray(nbrs, np.uint32)
+        voff = np.ascontiguousarray(voff, np.uint32)
+        h = C.c_void_p()
+        _check(lib().gpr_batch_create(_ptr(index), _ptr(nbrs) if len(nbrs) else None, _ptr(voff), len(voff) - 1,
+                                      C.byref(h)))
+        return cls(h)
+
+    def download(self):
+        index = np.empty(self.n + 1, np.uint64)
+        nbrs = np.empty(max(1, self.m), np.uint32)
+        voff = np.empty(self.count + 1, np.uint32)
+        _check(lib().gpr_batch_download(self.h, _ptr(index), _ptr(nbrs), _ptr(voff)))
+        return index, nbrs[: self.m], voff
+
+    def cc(self):
+        out = np.empty(max(1, self.n), np.uint32)
+        info = RunInfo()
+        _check(lib().gpr_batch_cc(self.h, _ptr(out), C.byref(info)))
+        return out[: self.n], info
+
+    def destroy(self):
+        if self.h:
+            _check(lib().gpr_batch_destroy(self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.destroy()
 
 
 _NP = {U32: np.uint32, F32: np.float32, F64: np.float64, U64: np.uint64, U16: np.uint16}

@@ -392,6 +392,16 @@ k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__re
 }
 
 // ---- host side ----------------------------------------------------------------
+// Degree prefix of a vertex list into g->scan / g->scan_blk (pos(i) =
+// scan[i] + scan_blk[i / SCAN_BLOCK]).
+int scan_frontier_degrees(gpr_graph *g, const uint32_t *list, uint32_t k) {
+    const uint32_t nb = (k + SCAN_BLOCK - 1) / SCAN_BLOCK;
+    if (nb) hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, list, k, g->deg, g->scan, g->scan_blk);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
 static unsigned pull_grid(uint32_t n) {
     uint32_t groups = (n + 63) / 64;
     return grid_for(groups, PULL_THREADS / 64, 2048);

@@ -195,6 +195,11 @@ static int rmat_impl(uint32_t scale, uint32_t ef, double a, double b, double c, 
     return GPR_OK;
 }
 
+int keys_to_csr_ext(const uint64_t *keys, uint64_t m, uint32_t scale, uint32_t n, uint32_t *nbrs, uint64_t *index,
+                    hipStream_t st) {
+    return keys_to_csr(keys, m, scale, n, nbrs, index, st);
+}
+
 // ---- degree-sorted relabelling ------------------------------------------------
 // Graptor's default graph type renumbers vertices (VEBO, graph/VEBOReorder.h:
 // 134-209) and maps results back with remapID/originalID

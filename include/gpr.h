@@ -257,6 +257,35 @@ typedef struct { int contrib; double tol; } gpr_pr_stage;
 int gpr_pr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages,
                 const gpr_run_opts *o, float *x, gpr_run_info *info);
 
+/* Multi-source BFSLVLv: k <= 64 sources (original IDs) as the bit lanes of
+ * one traversal; levels[j*n + v] is the level of v from sources[j] (n+1
+ * unreached), each lane equal to gpr_bfslvl from that source.  No
+ * counterpart in the reference, which runs one source per call
+ * (BFSLVLv.C:65, -start). */
+int gpr_msbfs(gpr_graph *g, const uint32_t *sources, int k,
+              const gpr_run_opts *o, uint32_t *levels, gpr_run_info *info);
+
+/* ---- batches of small graphs: CC over all of them in one persistent
+ *      launch (no counterpart in the reference; labels are CCv's canonical
+ *      form, CCv.C:121-169, in the batch's global vertex IDs) ------------- */
+typedef struct gpr_batch gpr_batch;
+/* count graphs, vertex counts log-uniform in [nmin, nmax], even graphs RMAT
+ * (edge factor 8), odd graphs Erdos-Renyi (mean degree 6), from seed. */
+int gpr_batch_generate(uint32_t count, uint32_t nmin, uint32_t nmax,
+                       uint64_t seed, gpr_batch **out);
+/* A CSR over the disjoint union (global IDs) with vertex offsets voff
+ * (count+1); every edge must stay inside its graph. */
+int gpr_batch_create(const uint64_t *index, const uint32_t *nbrs,
+                     const uint32_t *voff, uint32_t count, gpr_batch **out);
+int gpr_batch_info(const gpr_batch *b, uint32_t *count, uint32_t *n,
+                   uint64_t *m);
+int gpr_batch_download(const gpr_batch *b, uint64_t *index, uint32_t *nbrs,
+                       uint32_t *voff);
+/* labels[v] = minimum global vertex ID of v's component.  Runs as one
+ * hipGraph launch (queue reset + persistent kernel). */
+int gpr_batch_cc(gpr_batch *b, uint32_t *labels, gpr_run_info *info);
+int gpr_batch_destroy(gpr_batch *b);
+
 /* Per-iteration trace of the last run (-itimes, BFSLVLv.C:90-106). */
 int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
 

@@ -46,6 +46,9 @@ def lib():
         L.orc_rmat_thresholds.argtypes = [C.c_double, C.c_double, C.c_double, u64p]
         L.orc_rmat_edges.argtypes = [C.c_uint32, u64p, C.c_uint64, C.c_uint64, C.c_uint64, u32p, u32p]
         L.orc_er_edges.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p]
+        L.orc_batch_sizes.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, u32p]
+        L.orc_batch_graph.restype = C.c_uint64
+        L.orc_batch_graph.argtypes = [C.c_uint32, C.c_uint32, C.c_uint64, u64p, u32p]
         L.orc_build_csr.restype = C.c_uint64
         L.orc_build_csr.argtypes = [C.c_uint32, C.c_uint64, u32p, u32p, C.c_int, u64p, u32p]
         L.orc_b2_write.argtypes = [C.c_char_p, C.c_uint32, C.c_uint64, u64p, u32p]
@@ -144,6 +147,27 @@ def grid_graph(w, h, long_frac=0.0, seed=1):
     src = np.concatenate(src).astype(np.uint32)
     dst = np.concatenate(dst).astype(np.uint32)
     return build_csr(n, src, dst, True)
+
+
+def batch_graphs(count, nmin, nmax, seed):
+    """The batch of small graphs as one CSR with global IDs plus vertex
+    offsets (index, nbrs, voff)."""
+    nv = np.empty(count, np.uint32)
+    lib().orc_batch_sizes(count, nmin, nmax, seed, nv)
+    voff = np.zeros(count + 1, np.uint64)
+    voff[1:] = np.cumsum(nv.astype(np.uint64))
+    idx_parts, nb_parts = [np.zeros(1, np.uint64)], []
+    ebase = 0
+    for g in range(count):
+        n = int(nv[g])
+        index = np.zeros(n + 1, np.uint64)
+        nbrs = np.empty(16 * n + 16, np.uint32)
+        m = lib().orc_batch_graph(g, n, seed, index, nbrs)
+        idx_parts.append(index[1:] + np.uint64(ebase))
+        nb_parts.append(nbrs[:m] + np.uint32(voff[g]))
+        ebase += m
+    return (np.concatenate(idx_parts), np.concatenate(nb_parts) if nb_parts else np.zeros(0, np.uint32),
+            voff.astype(np.uint32))
 
 
 def b2_write(path, index, nbrs):

@@ -72,6 +72,62 @@ void orc_er_edges(uint32_t n, uint64_t seed, uint64_t count,
     }
 }
 
+/* ---- batches of small graphs (the batched CC workload) -------------------
+ * Graph g of a batch: n_g log-uniform in [nmin, nmax] from
+ * u = (mix64(seed ^ mix64(0x517e0000 + g)) >> 11) * 2^-53,
+ * n_g = floor(exp(log nmin + u (log nmax - log nmin))); graph seed
+ * gs = mix64(seed ^ mix64(0x5eed0000 + g)); even g: RMAT (a=.57 b=.19
+ * c=.19) at scale ceil(log2 n_g), 8 n_g draws keyed by gs, endpoints >= n_g
+ * dropped; odd g: 3 n_g uniform pairs h = mix64(gs ^ mix64(e)). */
+void orc_batch_sizes(uint32_t count, uint32_t nmin, uint32_t nmax, uint64_t seed, uint32_t *nv) {
+    const double l0 = log((double)nmin), l1 = log((double)nmax);
+    for (uint32_t g = 0; g < count; ++g) {
+        const double u = (double)(orc_mix64(seed ^ orc_mix64(0x517e0000ULL + g)) >> 11) * (1.0 / 9007199254740992.0);
+        double x = floor(exp(l0 + u * (l1 - l0)));
+        if (x < nmin) x = nmin;
+        if (x > nmax) x = nmax;
+        nv[g] = (uint32_t)x;
+    }
+}
+
+/* Local CSR of graph g (index: nv+1 entries, nbrs: capacity 16*nv). */
+uint64_t orc_batch_graph(uint32_t g, uint32_t nv, uint64_t seed, uint64_t *index, uint32_t *nbrs) {
+    const uint64_t gs = orc_mix64(seed ^ orc_mix64(0x5eed0000ULL + g));
+    const uint64_t cnt = (g & 1) == 0 ? 8ULL * nv : 3ULL * nv;
+    uint32_t *src = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
+    uint32_t *dst = (uint32_t *)malloc(sizeof(uint32_t) * cnt);
+    uint64_t k = 0;
+    if ((g & 1) == 0) {
+        uint32_t sc = 1;
+        while ((1u << sc) < nv) ++sc;
+        uint64_t t[3];
+        orc_rmat_thresholds(0.57, 0.19, 0.19, t);
+        for (uint64_t e = 0; e < cnt; ++e) {
+            uint32_t u = 0, v = 0;
+            uint64_t h = 0;
+            for (uint32_t l = 0; l < sc; ++l) {
+                if ((l & 1) == 0) h = orc_mix64(gs ^ orc_mix64((e << 5) | (l >> 1)));
+                uint32_t r = (l & 1) ? (uint32_t)(h >> 32) : (uint32_t)h;
+                uint32_t q = r < t[0] ? 0u : r < t[1] ? 1u : r < t[2] ? 2u : 3u;
+                u = (u << 1) | (q >> 1);
+                v = (v << 1) | (q & 1u);
+            }
+            if (u < nv && v < nv) { src[k] = u; dst[k] = v; ++k; }
+        }
+    } else {
+        for (uint64_t e = 0; e < cnt; ++e) {
+            uint64_t h = orc_mix64(gs ^ orc_mix64(e));
+            src[k] = (uint32_t)(((h & 0xffffffffULL) * (uint64_t)nv) >> 32);
+            dst[k] = (uint32_t)(((h >> 32) * (uint64_t)nv) >> 32);
+            ++k;
+        }
+    }
+    uint64_t m = orc_build_csr(nv, k, src, dst, 1, index, nbrs);
+    free(src);
+    free(dst);
+    return m;
+}
+
 /* LSD radix sort of 64-bit keys, 16 bits per pass, passes with a single
  * populated bucket skipped. */
 static void radix_sort_u64(uint64_t *a, uint64_t *tmp, size_t len) {

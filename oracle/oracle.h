@@ -38,6 +38,10 @@ void orc_rmat_edges(uint32_t scale, const uint64_t t[3], uint64_t seed,
 void orc_er_edges(uint32_t n, uint64_t seed, uint64_t count,
                   uint32_t *src, uint32_t *dst);
 
+/* Batches of small graphs: sizes and the local CSR of graph g. */
+void orc_batch_sizes(uint32_t count, uint32_t nmin, uint32_t nmax, uint64_t seed, uint32_t *nv);
+uint64_t orc_batch_graph(uint32_t g, uint32_t nv, uint64_t seed, uint64_t *index, uint32_t *nbrs);
+
 /* Clean an edge list into a CSR: drop self loops, (optionally) add the
  * reverse of every edge, sort and remove duplicates, keep isolated vertices.
  * Semantics of tools/CleanGraph.C:10-139 (without --rm-isolated),

@@ -329,3 +329,16 @@ def test_planted_mistakes_are_caught():
     i1, n1 = O.build_csr(256, src_e, dst_e, False)
     i2, n2 = brute_csr(256, src_e, dst_e, True)
     assert not (np.array_equal(i1, i2) and np.array_equal(n1, n2))
+
+
+def test_batch_sizes_and_graphs():
+    index, nbrs, voff = O.batch_graphs(200, 100, 5000, 7)
+    nv = np.diff(voff.astype(np.int64))
+    assert (nv >= 100).all() and (nv <= 5000).all()
+    # log-uniform: the median of log n sits near the middle of the range
+    mid = 0.5 * (np.log(100) + np.log(5000))
+    assert abs(np.median(np.log(nv)) - mid) < 0.6
+    assert O.check_csr(index, nbrs) == 0
+    src = np.repeat(np.arange(len(index) - 1), np.diff(index).astype(np.int64))
+    g_of = np.searchsorted(voff, src, side="right") - 1
+    assert (nbrs >= voff[g_of]).all() and (nbrs < voff[g_of + 1]).all()    # no cross-graph edge
