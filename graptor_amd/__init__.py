@@ -119,6 +119,13 @@ SIGNATURES = {
     "gpr_batch_download": (C.c_int, [_P, _P, _P, _P]),
     "gpr_batch_cc": (C.c_int, [_P, _P, C.POINTER(RunInfo)]),
     "gpr_batch_destroy": (C.c_int, [_P]),
+    "gpr_mlp_param_count": (C.c_int, []),
+    "gpr_policy_set": (C.c_int, [_P, _P, C.c_int]),
+    "gpr_policy_reset": (C.c_int, [_P]),
+    "gpr_policy_trace": (C.c_int, [_P, C.c_int, C.c_uint32, C.c_int, _P, _P, C.POINTER(C.c_int)]),
+    "gpr_mlp_forward": (C.c_int, [_P, _P, C.c_int, _P]),
+    "gpr_mlp_grad": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, C.POINTER(C.c_float)]),
+    "gpr_mlp_train": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]),
     "gpr_trace_get": (C.c_int, [_P, C.POINTER(IterStats), C.c_int, C.POINTER(C.c_int)]),
     "gpr_profile_enable": (C.c_int, [_P, C.c_int]),
     "gpr_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -290,6 +297,22 @@ class Graph:
         _check(lib().gpr_msbfs(self.h, _ptr(src), len(src), C.byref(o), _ptr(out), C.byref(info)))
         return out, info
 
+    def set_policy(self, params):
+        """Load the direction/representation network (None: threshold rule)."""
+        if params is None:
+            _check(lib().gpr_policy_set(self.h, None, 0))
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        _check(lib().gpr_policy_set(self.h, _ptr(p), len(p)))
+
+    def policy_trace(self, algo, start=0, cap=4096):
+        """Per-iteration features (k,6) and timed choices (k,4)."""
+        f = np.empty((cap, 6), np.float32)
+        t = np.empty((cap, 4), np.float32)
+        k = C.c_int()
+        _check(lib().gpr_policy_trace(self.h, algo, start, cap, _ptr(f), _ptr(t), C.byref(k)))
+        return f[: k.value].copy(), t[: k.value].copy()
+
     def trace(self):
         k = C.c_int()
         _check(lib().gpr_trace_get(self.h, None, 0, C.byref(k)))
@@ -450,6 +473,41 @@ class Frontier:
         if self.h:
             _check(lib().gpr_frontier_destroy(self.h))
             self.h = None
+
+
+def mlp_param_count():
+    return load_library().gpr_mlp_param_count()
+
+
+def mlp_forward(params, x):
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    out = np.empty((len(x), 4), np.float32)
+    _check(lib().gpr_mlp_forward(_ptr(p), _ptr(x), len(x), _ptr(out)))
+    return out
+
+
+def mlp_grad(params, x, targets, weights=None):
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    t = np.ascontiguousarray(targets, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    g = np.empty(len(p), np.float32)
+    loss = C.c_float()
+    _check(lib().gpr_mlp_grad(_ptr(p), _ptr(x), _ptr(t), _ptr(w) if w is not None else None, len(x), _ptr(g),
+                              C.byref(loss)))
+    return g, loss.value
+
+
+def mlp_train(params, x, targets, weights=None, epochs=500, lr=1e-2):
+    x = np.ascontiguousarray(x, np.float32)
+    p = np.array(params, np.float32)
+    t = np.ascontiguousarray(targets, np.float32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.float32)
+    loss = C.c_float()
+    _check(lib().gpr_mlp_train(_ptr(p), _ptr(x), _ptr(t), _ptr(w) if w is not None else None, len(x), epochs, lr,
+                               C.byref(loss)))
+    return p, loss.value
 
 
 def version():

@@ -78,6 +78,7 @@ struct BFSLVLv {
         A::make_lazy_executor(GA).vertex_map_fill(a_level, n + 1).materialize();
         A::frontier F = A::frontier::create(GA, start);
         A::make_lazy_executor(GA).vertex_map_fill(F, a_level, 0).materialize();
+        check(gpr_policy_reset(GA.get()), "policy reset");
         info->edges += F.nActiveEdges();
         iter = 0;
         while (!F.isEmpty()) {
@@ -139,6 +140,7 @@ struct CCv {
         const uint32_t n = GA.numVertices();
         // CCv.C:246-248 IDs[v] = v
         A::make_lazy_executor(GA).vertex_map_iota(m_IDs).materialize();
+        check(gpr_policy_reset(GA.get()), "policy reset");
         iter = 0;
         uint32_t initial_zeros = 1;
         // PRESET_ZERO, CCv.C:262-291: IDs[N(0)] = 0, vertex 0 of the

@@ -26,6 +26,7 @@
 //         step, wave-min, early exit once the destination filter fails).
 #include "internal.hpp"
 #include "device.hpp"
+#include "policy_dev.hpp"
 
 namespace gpr {
 
@@ -102,6 +103,31 @@ template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) 
     return RELAX == GPR_RELAX_MIN_PLUS1 ? ps + 1u : ps;
 }
 
+// Fused policy inference: the last workgroup to finish (by ticket) turns
+// the completed frontier counts into the next step's choice.
+__device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounters *ctr) {
+    if (!pol.params) return;
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&ctr->ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (last && threadIdx.x < 64) {
+        __threadfence();
+        const uint32_t nv = __hip_atomic_load(&ctr->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t ne = __hip_atomic_load(&ctr->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float f[6];
+        policy_features(pol, nv, ne, f);
+        const int d = policy_forward_wave(pol.params, f);
+        if (threadIdx.x == 0) {
+            ctr->decision = (unsigned)d;
+            ctr->ticket = 0;
+        }
+    }
+}
+
 // ---- push ------------------------------------------------------------------
 constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ITEMS;
 constexpr int PUSH_LDS = PUSH_TILE + 1;
@@ -111,7 +137,7 @@ __global__ void __launch_bounds__(PUSH_THREADS)
 k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restrict__ pos,
        const uint64_t *__restrict__ btot, uint64_t E, const uint64_t *__restrict__ index,
        const uint32_t *__restrict__ nbrs, const uint32_t *__restrict__ deg, uint32_t *prop, int dkind,
-       uint32_t darg, uint64_t *bits, uint32_t *out_list, DevCounters *ctr) {
+       uint32_t darg, uint64_t *bits, uint32_t *out_list, DevCounters *ctr, PolicyCtx pol) {
     __shared__ uint64_t s_pos[PUSH_LDS];
     __shared__ uint32_t s_v[PUSH_LDS];
     __shared__ uint32_t q[PUSH_TILE];
@@ -196,6 +222,7 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
         if (tv) atomicAdd(&ctr->nactv, tv);
         if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
     }
+    policy_epilogue(pol, ctr);
 }
 
 // ---- pull ------------------------------------------------------------------
@@ -212,7 +239,7 @@ __global__ void __launch_bounds__(PULL_THREADS)
 k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
        const uint32_t *__restrict__ deg, uint32_t *prop, const uint32_t *__restrict__ prev, int dkind,
        uint32_t darg, int record, const uint64_t *__restrict__ in_bits, uint64_t *out_bits,
-       uint32_t *out_list, DevCounters *ctr) {
+       uint32_t *out_list, DevCounters *ctr, PolicyCtx pol) {
     __shared__ uint64_t red[PULL_THREADS / 64];
     __shared__ uint32_t red32[PULL_THREADS / 64];
     const uint32_t ngroups = (n + 63) / 64;
@@ -313,6 +340,7 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
         if (ts) atomicAdd(&ctr->edges, (unsigned long long)ts);
     }
+    policy_epilogue(pol, ctr);
 }
 
 // ---- vertex maps -------------------------------------------------------------
@@ -409,6 +437,8 @@ static unsigned pull_grid(uint32_t n) {
 
 int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st);
 int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *out_kind);
+PolicyCtx policy_ctx(gpr_graph *g, uint64_t nacte_in);
+void policy_after_step(gpr_graph *g, uint64_t nacte_in, uint32_t decision);
 
 int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_frontier *out,
                      gpr_iter_stats *st) {
@@ -433,7 +463,10 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     }
     int dir = desc->direction;
     int okind = desc->out_kind;
+    const bool use_pol = dir == GPR_DIR_POLICY && g->pol_params;
     if (dir == GPR_DIR_POLICY) GPR_TRY(policy_decide(g, nactv, nacte, &dir, &okind));
+    PolicyCtx pol = {};
+    if (use_pol) pol = policy_ctx(g, nacte);
     if (dir == GPR_DIR_AUTO) {
         const float pct = desc->threshold_pct > 0.f ? desc->threshold_pct : 5.f;
         // frac_threshold::is_sparse, api.h:135-139
@@ -477,11 +510,11 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
             if (desc->relax == GPR_RELAX_MIN_PLUS1)
                 hipLaunchKernelGGL(k_push<GPR_RELAX_MIN_PLUS1>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
                                    g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
-                                   desc->dst_arg, bits, olist, g->ctr);
+                                   desc->dst_arg, bits, olist, g->ctr, pol);
             else
                 hipLaunchKernelGGL(k_push<GPR_RELAX_MIN>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
                                    g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
-                                   desc->dst_arg, bits, olist, g->ctr);
+                                   desc->dst_arg, bits, olist, g->ctr, pol);
         }
         if (olist) GPR_TRY(clear_dedup_of_list(g, olist, E < n ? E : n));
         s.edges_scanned = E;
@@ -493,11 +526,11 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         if (desc->relax == GPR_RELAX_MIN_PLUS1)
             hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN_PLUS1>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n,
                                g->cindex, g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits,
-                               ob, ol, g->ctr);
+                               ob, ol, g->ctr, pol);
         else
             hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex,
                                g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol,
-                               g->ctr);
+                               g->ctr, pol);
     }
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipEventRecord(g->ev1, g->stream));
@@ -513,6 +546,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         s.nacte_out = out->nacte;
     }
     if (dir == GPR_DIR_PULL) s.edges_scanned = g->hctr->edges;
+    if (use_pol) policy_after_step(g, nacte, g->hctr->decision);
     s.direction = dir;
     s.out_kind = okind;
     s.ms = ms;

@@ -73,7 +73,7 @@ static int build_partitions(gpr_graph *g, const uint64_t *cindex) {
 }
 
 // Out-degree maximum, ties to the lowest vertex ID.
-static uint32_t highest_degree(const uint64_t *index, uint32_t n) {
+uint32_t highest_degree(const uint64_t *index, uint32_t n) {
     uint64_t best = 0;
     uint32_t v = 0;
     for (uint32_t r = 0; r < n; ++r) {
@@ -92,6 +92,7 @@ static int finish_graph(gpr_graph *g, const uint64_t *host_index, const uint64_t
     GPR_HIP(hipGetLastError());
     GPR_TRY(build_partitions(g, host_cindex));
     g->hdv = highest_degree(host_index, n);
+    g->max_deg = n ? (uint32_t)(host_index[g->hdv + 1] - host_index[g->hdv]) : 0;
     GPR_TRY(dev_alloc((void **)&g->ctr, sizeof(DevCounters), "counters"));
     GPR_HIP(hipMemset(g->ctr, 0, sizeof(DevCounters)));
     GPR_HIP(hipHostMalloc((void **)&g->hctr, sizeof(DevCounters), hipHostMallocDefault));
@@ -132,6 +133,7 @@ static void graph_free(gpr_graph *g) {
     dev_free(g->partial_f);
     dev_free(g->dsum_slices);
     dev_free(g->prs);
+    dev_free(g->pol_params);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->rev0) hipEventDestroy(g->rev0);

@@ -67,6 +67,8 @@ struct DevCounters {
     unsigned int listpos;       // append position of the list being produced
     unsigned int ticket;        // last-workgroup tickets (reset by that wg)
     unsigned int ticket2;
+    unsigned int decision;      // policy choice for the next step
+    unsigned int pad;
 };
 
 // Scalars of the PageRank iteration, kept on the device so that an
@@ -132,6 +134,13 @@ struct gpr_graph {
     hipEvent_t rev0 = nullptr, rev1 = nullptr;  // per application run
     // recycled frontier buffers (avoid hipMalloc per iteration)
     std::vector<void *> pool_list, pool_bits;
+    // learned policy state of the current run
+    float *pol_params = nullptr;    // device copy of the network (null: none)
+    bool pol_have = false;          // a device decision for the next step exists
+    int pol_next = 0;
+    int pol_iter = 0;
+    double pol_unvisited = 0;
+    uint32_t max_deg = 0;
 };
 
 struct gpr_prop {

@@ -1,15 +1,520 @@
-// policy.hip -- per-iteration choice of direction and frontier
-// representation.  The fallback is the reference's threshold rule
-// (default_threshold::is_sparse, api.h:150-153).
+// policy.hip -- the per-iteration choice of traversal direction and output
+// frontier representation.
+//
+// Fallback and judge: the reference's threshold rule (sparse push iff
+// nactv + nacte <= m/20, api.h:150-153; a list output for push and a dense
+// output for pull, dsl/edgemap.h:1571-1601).  Learned policy: a 6-32-32-4
+// MLP (policy_dev.hpp) whose forward pass runs in the last workgroup of each
+// push/pull kernel; the host reads its choice together with the frontier
+// counts it already reads.  Outputs are identical whichever choice is made
+// (the relax is a min, the frontier a set); only the time differs.
+//
+// Training data: gpr_policy_trace runs BFS or CC and, at every iteration,
+// times all four choices from a snapshot of the state, each followed by the
+// better of push/pull for the next step (so the cost of the representation
+// handed to the next step counts), then advances with the fastest choice.
+// Training: full-batch Adam on the GPU with this file's forward/backward
+// kernels (per-sample soft targets and weights from the timings).
 #include "internal.hpp"
+#include "device.hpp"
+#include "policy_dev.hpp"
+
+#include <math.h>
+#include <string.h>
+
+#include <cmath>
+
+#include <algorithm>
+#include <vector>
 
 namespace gpr {
 
+// ---- host mirror of the feature map and the forward pass --------------------
+static void host_features(const gpr_graph *g, uint32_t nactv, uint64_t nacte, double unvisited, int iter, float f[6]) {
+    const double mean = g->n ? (double)g->m / (double)g->n : 0.0;
+    f[0] = log2f(1.0f + (float)nactv) / 32.0f;
+    f[1] = log2f(1.0f + (float)nacte) / 40.0f;
+    f[2] = (float)log2(1.0 + (unvisited > 0 ? unvisited : 0.0)) / 40.0f;
+    f[3] = (float)iter / 64.0f;
+    f[4] = (float)log2(1.0 + mean) / 8.0f;
+    f[5] = (float)log2(1.0 + (mean > 0 ? g->max_deg / mean : 0.0)) / 24.0f;
+}
+
+static int host_forward(const float *p, const float f[6]) {
+    const MlpView P(p);
+    float h1[MLP_H], h2[MLP_H], z[MLP_OUT];
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P.b1[j];
+        for (int i = 0; i < MLP_IN; ++i) a += f[i] * P.w1[i * MLP_H + j];
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P.b2[j];
+        for (int i = 0; i < MLP_H; ++i) a += h1[i] * P.w2[i * MLP_H + j];
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    int best = 0;
+    for (int k = 0; k < MLP_OUT; ++k) {
+        float a = P.b3[k];
+        for (int i = 0; i < MLP_H; ++i) a += h2[i] * P.w3[i * MLP_OUT + k];
+        z[k] = a;
+        if (z[k] > z[best]) best = k;
+    }
+    return best;
+}
+
+PolicyCtx policy_ctx(gpr_graph *g, uint64_t nacte_in) {
+    PolicyCtx c = {};
+    c.params = g->pol_params;
+    const double mean = g->n ? (double)g->m / (double)g->n : 0.0;
+    c.mean_deg_feat = (float)log2(1.0 + mean) / 8.0f;
+    c.skew_feat = (float)log2(1.0 + (mean > 0 ? g->max_deg / mean : 0.0)) / 24.0f;
+    c.iter_feat = (float)(g->pol_iter + 1) / 64.0f;
+    c.unvisited_before = g->pol_unvisited - (double)nacte_in;
+    return c;
+}
+
+// Choice for the current step.  No network loaded: the threshold rule.
 int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *out_kind) {
-    const bool sparse = (uint64_t)nactv + nacte <= g->m / 20;
-    *dir = sparse ? GPR_DIR_PUSH : GPR_DIR_PULL;
-    if (*out_kind == 0) *out_kind = sparse ? GPR_FRONTIER_LIST : GPR_FRONTIER_BITMAP;
+    int d;
+    if (!g->pol_params) {
+        const bool sparse = (uint64_t)nactv + nacte <= g->m / 20;
+        *dir = sparse ? GPR_DIR_PUSH : GPR_DIR_PULL;
+        if (*out_kind == 0) *out_kind = sparse ? GPR_FRONTIER_LIST : GPR_FRONTIER_BITMAP;
+        return GPR_OK;
+    }
+    if (g->pol_have) {
+        d = g->pol_next;
+    } else {
+        std::vector<float> p(MLP_PARAMS);
+        GPR_HIP(hipMemcpy(p.data(), g->pol_params, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost));
+        float f[6];
+        host_features(g, nactv, nacte, g->pol_unvisited - (double)nacte, g->pol_iter, f);
+        d = host_forward(p.data(), f);
+    }
+    *dir = d < 2 ? GPR_DIR_PUSH : GPR_DIR_PULL;
+    *out_kind = (d & 1) ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST;
+    return GPR_OK;
+}
+
+// Book-keeping after a step run under the policy.
+void policy_after_step(gpr_graph *g, uint64_t nacte_in, uint32_t decision) {
+    g->pol_unvisited -= (double)nacte_in;
+    if (g->pol_unvisited < 0) g->pol_unvisited = 0;
+    g->pol_iter++;
+    g->pol_next = (int)decision;
+    g->pol_have = true;
+}
+
+// ---- frontier snapshots ------------------------------------------------------------
+struct FrontierCopy {
+    int kind = GPR_FRONTIER_EMPTY;
+    uint32_t nactv = 0;
+    uint64_t nacte = 0;
+    uint32_t *list = nullptr;
+    uint64_t *bits = nullptr;
+};
+
+static int save_frontier(gpr_graph *g, const gpr_frontier *f, FrontierCopy &c) {
+    c.kind = f->kind;
+    c.nactv = f->nactv;
+    c.nacte = f->nacte;
+    if (f->kind == GPR_FRONTIER_LIST && f->nactv) {
+        if (!c.list) GPR_TRY(dev_alloc((void **)&c.list, 4ull * std::max<uint32_t>(g->n, 1), "snapshot"));
+        GPR_HIP(hipMemcpyAsync(c.list, f->list, 4ull * f->nactv, hipMemcpyDeviceToDevice, g->stream));
+    }
+    if (f->kind == GPR_FRONTIER_BITMAP) {
+        if (!c.bits) GPR_TRY(dev_alloc((void **)&c.bits, 8ull * nwords(g->n), "snapshot"));
+        GPR_HIP(hipMemcpyAsync(c.bits, f->bits, 8ull * nwords(g->n), hipMemcpyDeviceToDevice, g->stream));
+    }
+    return GPR_OK;
+}
+
+static int restore_frontier(gpr_graph *g, gpr_frontier *f, const FrontierCopy &c) {
+    f->kind = c.kind;
+    f->nactv = c.nactv;
+    f->nacte = c.nacte;
+    if (c.kind == GPR_FRONTIER_LIST && c.nactv) {
+        GPR_TRY(frontier_ensure_list(f));
+        GPR_HIP(hipMemcpyAsync(f->list, c.list, 4ull * c.nactv, hipMemcpyDeviceToDevice, g->stream));
+    }
+    if (c.kind == GPR_FRONTIER_BITMAP) {
+        GPR_TRY(frontier_ensure_bits(f));
+        GPR_HIP(hipMemcpyAsync(f->bits, c.bits, 8ull * nwords(g->n), hipMemcpyDeviceToDevice, g->stream));
+    }
+    return GPR_OK;
+}
+
+static void free_copy(FrontierCopy &c) {
+    dev_free(c.list);
+    dev_free(c.bits);
+    c.list = nullptr;
+    c.bits = nullptr;
+}
+
+// ---- MLP training kernels ------------------------------------------------------------
+constexpr int MB = 64;   // samples per workgroup
+
+// Forward + backward of MB samples per workgroup.  Loss per sample:
+// w * sum_k -t_k log softmax(z)_k.  Writes the workgroup's parameter
+// gradient (sum over its samples) and loss to gpart/lpart.
+__global__ void __launch_bounds__(MB)
+k_mlp_fwd_bwd(const float *__restrict__ params, const float *__restrict__ X, const float *__restrict__ T,
+              const float *__restrict__ Wt, int count, float *gpart, float *lpart, float *logits) {
+    __shared__ float sx[MB][MLP_IN];
+    __shared__ float sh1[MB][MLP_H], sh2[MB][MLP_H], sd1[MB][MLP_H], sd2[MB][MLP_H];
+    __shared__ float sdz[MB][MLP_OUT];
+    __shared__ float sloss[MB];
+    const MlpView P(params);
+    const int t = threadIdx.x;
+    const int s = blockIdx.x * MB + t;
+    const bool valid = s < count;
+    float x[MLP_IN], h1[MLP_H], h2[MLP_H], z[MLP_OUT];
+    for (int i = 0; i < MLP_IN; ++i) x[i] = valid ? X[(size_t)s * MLP_IN + i] : 0.f;
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P.b1[j];
+        for (int i = 0; i < MLP_IN; ++i) a += x[i] * P.w1[i * MLP_H + j];
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = P.b2[j];
+        for (int i = 0; i < MLP_H; ++i) a += h1[i] * P.w2[i * MLP_H + j];
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    float zmax = -3.0e38f;
+    for (int k = 0; k < MLP_OUT; ++k) {
+        float a = P.b3[k];
+        for (int i = 0; i < MLP_H; ++i) a += h2[i] * P.w3[i * MLP_OUT + k];
+        z[k] = a;
+        zmax = fmaxf(zmax, a);
+        if (logits && valid) logits[(size_t)s * MLP_OUT + k] = a;
+    }
+    float se = 0.f;
+    for (int k = 0; k < MLP_OUT; ++k) se += expf(z[k] - zmax);
+    const float lse = zmax + logf(se);
+    const float w = valid && Wt ? Wt[s] : (valid ? 1.f : 0.f);
+    float loss = 0.f, dz[MLP_OUT];
+    for (int k = 0; k < MLP_OUT; ++k) {
+        const float tk = valid && T ? T[(size_t)s * MLP_OUT + k] : 0.f;
+        loss += -tk * (z[k] - lse);
+        dz[k] = w * (expf(z[k] - lse) - tk);   // targets sum to 1
+    }
+    loss *= w;
+    float d2[MLP_H], d1[MLP_H];
+    for (int j = 0; j < MLP_H; ++j) {
+        float a = 0.f;
+        for (int k = 0; k < MLP_OUT; ++k) a += dz[k] * P.w3[j * MLP_OUT + k];
+        d2[j] = h2[j] > 0.f ? a : 0.f;
+    }
+    for (int i = 0; i < MLP_H; ++i) {
+        float a = 0.f;
+        for (int j = 0; j < MLP_H; ++j) a += d2[j] * P.w2[i * MLP_H + j];
+        d1[i] = h1[i] > 0.f ? a : 0.f;
+    }
+    for (int i = 0; i < MLP_IN; ++i) sx[t][i] = x[i];
+    for (int j = 0; j < MLP_H; ++j) {
+        sh1[t][j] = h1[j];
+        sh2[t][j] = h2[j];
+        sd1[t][j] = d1[j];
+        sd2[t][j] = d2[j];
+    }
+    for (int k = 0; k < MLP_OUT; ++k) sdz[t][k] = dz[k];
+    sloss[t] = loss;
+    __syncthreads();
+    // parameter gradients: each entry a sum over the workgroup's samples
+    float *gout = gpart + (size_t)blockIdx.x * MLP_PARAMS;
+    for (int p = t; p < MLP_PARAMS; p += MB) {
+        float acc = 0.f;
+        int q = p;
+        if (q < MLP_IN * MLP_H) {                         // W1[i][j]
+            const int i = q / MLP_H, j = q % MLP_H;
+            for (int r = 0; r < MB; ++r) acc += sx[r][i] * sd1[r][j];
+        } else if ((q -= MLP_IN * MLP_H) < MLP_H) {       // b1[j]
+            for (int r = 0; r < MB; ++r) acc += sd1[r][q];
+        } else if ((q -= MLP_H) < MLP_H * MLP_H) {        // W2[i][j]
+            const int i = q / MLP_H, j = q % MLP_H;
+            for (int r = 0; r < MB; ++r) acc += sh1[r][i] * sd2[r][j];
+        } else if ((q -= MLP_H * MLP_H) < MLP_H) {        // b2[j]
+            for (int r = 0; r < MB; ++r) acc += sd2[r][q];
+        } else if ((q -= MLP_H) < MLP_H * MLP_OUT) {      // W3[i][k]
+            const int i = q / MLP_OUT, k = q % MLP_OUT;
+            for (int r = 0; r < MB; ++r) acc += sh2[r][i] * sdz[r][k];
+        } else {                                          // b3[k]
+            q -= MLP_H * MLP_OUT;
+            for (int r = 0; r < MB; ++r) acc += sdz[r][q];
+        }
+        gout[p] = acc;
+    }
+    if (t == 0) {
+        float l = 0.f;
+        for (int r = 0; r < MB; ++r) l += sloss[r];
+        lpart[blockIdx.x] = l;
+    }
+}
+
+// Sum the workgroup gradients in order (deterministic), divide by the total
+// weight; mode 0: write the gradient; mode 1: Adam step.
+__global__ void __launch_bounds__(256)
+k_mlp_reduce(const float *__restrict__ gpart, const float *__restrict__ lpart, int nblocks, float wsum,
+             float *params, float *grad, float *mom, float *vel, float lr, int step, int mode, float *loss) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int p = threadIdx.x; p < MLP_PARAMS; p += blockDim.x) {
+        float g = 0.f;
+        for (int b = 0; b < nblocks; ++b) g += gpart[(size_t)b * MLP_PARAMS + p];
+        g /= wsum;
+        if (mode == 0) {
+            grad[p] = g;
+        } else {
+            const float m = b1 * mom[p] + (1.f - b1) * g;
+            const float v = b2 * vel[p] + (1.f - b2) * g * g;
+            mom[p] = m;
+            vel[p] = v;
+            const float mh = m / (1.f - powf(b1, (float)step));
+            const float vh = v / (1.f - powf(b2, (float)step));
+            params[p] -= lr * mh / (sqrtf(vh) + eps);
+        }
+    }
+    if (threadIdx.x == 0) {
+        float l = 0.f;
+        for (int b = 0; b < nblocks; ++b) l += lpart[b];
+        *loss = l / wsum;
+    }
+}
+
+static hipStream_t mlp_stream() {
+    static hipStream_t s = nullptr;
+    if (!s) (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    return s;
+}
+
+struct MlpBuffers {
+    float *p = nullptr, *x = nullptr, *t = nullptr, *w = nullptr, *gpart = nullptr, *lpart = nullptr;
+    float *grad = nullptr, *mom = nullptr, *vel = nullptr, *loss = nullptr, *logits = nullptr;
+    ~MlpBuffers() {
+        for (float *q : {p, x, t, w, gpart, lpart, grad, mom, vel, loss, logits}) dev_free(q);
+    }
+};
+
+static int mlp_upload(MlpBuffers &B, const float *params, const float *X, const float *T, const float *W, int count) {
+    const int nb = (count + MB - 1) / MB;
+    GPR_TRY(dev_alloc((void **)&B.p, 4ull * MLP_PARAMS, "mlp params"));
+    GPR_TRY(dev_alloc((void **)&B.x, 4ull * MLP_IN * std::max(count, 1), "mlp inputs"));
+    GPR_TRY(dev_alloc((void **)&B.gpart, 4ull * MLP_PARAMS * std::max(nb, 1), "mlp gradients"));
+    GPR_TRY(dev_alloc((void **)&B.lpart, 4ull * std::max(nb, 1), "mlp losses"));
+    GPR_TRY(dev_alloc((void **)&B.grad, 4ull * MLP_PARAMS, "mlp gradient"));
+    GPR_TRY(dev_alloc((void **)&B.loss, 4, "mlp loss"));
+    hipStream_t st = mlp_stream();
+    GPR_HIP(hipMemcpyAsync(B.p, params, 4ull * MLP_PARAMS, hipMemcpyHostToDevice, st));
+    GPR_HIP(hipMemcpyAsync(B.x, X, 4ull * MLP_IN * count, hipMemcpyHostToDevice, st));
+    if (T) {
+        GPR_TRY(dev_alloc((void **)&B.t, 4ull * MLP_OUT * std::max(count, 1), "mlp targets"));
+        GPR_HIP(hipMemcpyAsync(B.t, T, 4ull * MLP_OUT * count, hipMemcpyHostToDevice, st));
+    }
+    if (W) {
+        GPR_TRY(dev_alloc((void **)&B.w, 4ull * std::max(count, 1), "mlp weights"));
+        GPR_HIP(hipMemcpyAsync(B.w, W, 4ull * count, hipMemcpyHostToDevice, st));
+    }
     return GPR_OK;
 }
 
 } // namespace gpr
+
+using namespace gpr;
+
+extern "C" int gpr_policy_set(gpr_graph *g, const float *params, int count) {
+    if (!g) return GPR_E_INVAL;
+    if (!params) {
+        dev_free(g->pol_params);
+        g->pol_params = nullptr;
+        return GPR_OK;
+    }
+    GPR_REQUIRE(count == MLP_PARAMS, "policy: expected %d parameters, got %d", MLP_PARAMS, count);
+    for (int i = 0; i < count; ++i) GPR_REQUIRE(std::isfinite(params[i]), "policy: parameter %d is not finite", i);
+    if (!g->pol_params) GPR_TRY(dev_alloc((void **)&g->pol_params, 4ull * MLP_PARAMS, "policy"));
+    GPR_HIP(hipMemcpy(g->pol_params, params, 4ull * MLP_PARAMS, hipMemcpyHostToDevice));
+    return GPR_OK;
+}
+
+extern "C" int gpr_policy_reset(gpr_graph *g) {
+    if (!g) return GPR_E_INVAL;
+    g->pol_have = false;
+    g->pol_iter = 0;
+    g->pol_unvisited = (double)g->m;
+    return GPR_OK;
+}
+
+extern "C" int gpr_mlp_param_count(void) { return MLP_PARAMS; }
+
+extern "C" int gpr_mlp_forward(const float *params, const float *x, int count, float *logits) {
+    if (!params || !x || !logits || count <= 0) return GPR_E_INVAL;
+    MlpBuffers B;
+    GPR_TRY(mlp_upload(B, params, x, nullptr, nullptr, count));
+    GPR_TRY(dev_alloc((void **)&B.logits, 4ull * MLP_OUT * count, "mlp logits"));
+    const int nb = (count + MB - 1) / MB;
+    hipStream_t st = mlp_stream();
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nb), dim3(MB), 0, st, B.p, B.x, (const float *)nullptr,
+                       (const float *)nullptr, count, B.gpart, B.lpart, B.logits);
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(logits, B.logits, 4ull * MLP_OUT * count, hipMemcpyDeviceToHost, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    return GPR_OK;
+}
+
+extern "C" int gpr_mlp_grad(const float *params, const float *x, const float *targets, const float *weights,
+                            int count, float *grad, float *loss) {
+    if (!params || !x || !targets || !grad || count <= 0) return GPR_E_INVAL;
+    MlpBuffers B;
+    GPR_TRY(mlp_upload(B, params, x, targets, weights, count));
+    const int nb = (count + MB - 1) / MB;
+    float wsum = 0.f;
+    for (int i = 0; i < count; ++i) wsum += weights ? weights[i] : 1.f;
+    GPR_REQUIRE(wsum > 0.f, "mlp: weights sum to zero");
+    hipStream_t st = mlp_stream();
+    hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nb), dim3(MB), 0, st, B.p, B.x, B.t, B.w, count, B.gpart, B.lpart,
+                       (float *)nullptr);
+    hipLaunchKernelGGL(k_mlp_reduce, dim3(1), dim3(256), 0, st, B.gpart, B.lpart, nb, wsum, B.p, B.grad,
+                       (float *)nullptr, (float *)nullptr, 0.f, 0, 0, B.loss);
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(grad, B.grad, 4ull * MLP_PARAMS, hipMemcpyDeviceToHost, st));
+    float l = 0.f;
+    GPR_HIP(hipMemcpyAsync(&l, B.loss, 4, hipMemcpyDeviceToHost, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    return GPR_OK;
+}
+
+extern "C" int gpr_mlp_train(float *params, const float *x, const float *targets, const float *weights, int count,
+                             int epochs, float lr, float *loss) {
+    if (!params || !x || !targets || count <= 0 || epochs <= 0) return GPR_E_INVAL;
+    MlpBuffers B;
+    GPR_TRY(mlp_upload(B, params, x, targets, weights, count));
+    GPR_TRY(dev_alloc((void **)&B.mom, 4ull * MLP_PARAMS, "adam"));
+    GPR_TRY(dev_alloc((void **)&B.vel, 4ull * MLP_PARAMS, "adam"));
+    hipStream_t st = mlp_stream();
+    GPR_HIP(hipMemsetAsync(B.mom, 0, 4ull * MLP_PARAMS, st));
+    GPR_HIP(hipMemsetAsync(B.vel, 0, 4ull * MLP_PARAMS, st));
+    const int nb = (count + MB - 1) / MB;
+    float wsum = 0.f;
+    for (int i = 0; i < count; ++i) wsum += weights ? weights[i] : 1.f;
+    GPR_REQUIRE(wsum > 0.f, "mlp: weights sum to zero");
+    for (int e = 1; e <= epochs; ++e) {
+        hipLaunchKernelGGL(k_mlp_fwd_bwd, dim3(nb), dim3(MB), 0, st, B.p, B.x, B.t, B.w, count, B.gpart, B.lpart,
+                           (float *)nullptr);
+        hipLaunchKernelGGL(k_mlp_reduce, dim3(1), dim3(256), 0, st, B.gpart, B.lpart, nb, wsum, B.p, B.grad, B.mom,
+                           B.vel, lr, e, 1, B.loss);
+    }
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(params, B.p, 4ull * MLP_PARAMS, hipMemcpyDeviceToHost, st));
+    float l = 0.f;
+    GPR_HIP(hipMemcpyAsync(&l, B.loss, 4, hipMemcpyDeviceToHost, st));
+    GPR_HIP(hipStreamSynchronize(st));
+    if (loss) *loss = l;
+    return GPR_OK;
+}
+
+// Trace generation: algo 0 = BFSLVLv from `start`, 1 = CCv (all-true start,
+// no PRESET_ZERO).  features: k x 6, times: k x 4 (ms of the step plus the
+// better next step).
+extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap, float *features, float *times,
+                                int *k) {
+    if (!g || !k || cap <= 0 || (algo != 0 && algo != 1)) return GPR_E_INVAL;
+    GPR_REQUIRE(algo == 1 || start < g->n, "trace: start out of range");
+    const uint32_t n = g->n;
+    gpr_prop *P = nullptr, *Pb = nullptr, *Pb2 = nullptr;
+    gpr_frontier *F = nullptr, *O = nullptr, *O2 = nullptr;
+    FrontierCopy fs, os;
+    float *saved = g->pol_params;
+    g->pol_params = nullptr;   // the steps below force their choices
+    int cnt = 0;
+    int r = GPR_OK;
+    auto step = [&](gpr_frontier *in, gpr_frontier *out, int choice, int iter, gpr_iter_stats *st) -> int {
+        gpr_emap_desc d = {};
+        d.relax = algo == 0 ? GPR_RELAX_MIN_PLUS1 : GPR_RELAX_MIN;
+        d.prop = P;
+        d.dst_filter = algo == 0 ? GPR_DST_GT : GPR_DST_NE;
+        d.dst_arg = algo == 0 ? (uint32_t)iter + 1 : 0u;
+        d.record = GPR_RECORD_REDUCTION;
+        d.direction = choice < 2 ? GPR_DIR_PUSH : GPR_DIR_PULL;
+        d.out_kind = (choice & 1) ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST;
+        d.threshold_pct = 5.f;
+        return gpr_edgemap(g, &d, in, out, st);
+    };
+    do {
+        if ((r = gpr_prop_create(g, GPR_U32, &P))) break;
+        if ((r = gpr_prop_create(g, GPR_U32, &Pb))) break;
+        if ((r = gpr_prop_create(g, GPR_U32, &Pb2))) break;
+        if ((r = gpr_frontier_create(g, &F))) break;
+        if ((r = gpr_frontier_create(g, &O))) break;
+        if ((r = gpr_frontier_create(g, &O2))) break;
+        if (algo == 0) {
+            if ((r = gpr_vertex_map(g, GPR_VMAP_FILL, P, nullptr, nullptr, n + 1))) break;
+            if ((r = gpr_frontier_set_single(F, start))) break;
+            if ((r = gpr_vertex_map(g, GPR_VMAP_FILL_FRONTIER, P, nullptr, F, 0))) break;
+        } else {
+            if ((r = gpr_vertex_map(g, GPR_VMAP_IOTA, P, nullptr, nullptr, 0))) break;
+            if ((r = gpr_frontier_set_all(F))) break;
+        }
+        double U = (double)g->m;
+        int iter = 0;
+        while (F->nactv > 0 && F->kind != GPR_FRONTIER_EMPTY && cnt < cap) {
+            float f[6];
+            host_features(g, F->nactv, F->nacte, U - (double)F->nacte, iter, f);
+            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, Pb, P, nullptr, 0))) break;
+            if ((r = save_frontier(g, F, fs))) break;
+            double best = 1e30;
+            int bestc = 0;
+            for (int c = 0; c < 4 && !r; ++c) {
+                double tc = 1e30;
+                for (int rep = 0; rep < 2 && !r; ++rep) {
+                    if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb, nullptr, 0))) break;
+                    if ((r = restore_frontier(g, F, fs))) break;
+                    gpr_iter_stats st = {};
+                    if ((r = step(F, O, c, iter, &st))) break;
+                    double t = st.ms;
+                    if (O->nactv > 0 && O->kind != GPR_FRONTIER_EMPTY) {
+                        // lookahead: the better direction for the next step
+                        if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, Pb2, P, nullptr, 0))) break;
+                        if ((r = save_frontier(g, O, os))) break;
+                        double t2 = 1e30;
+                        for (int c2 : {0, 3}) {
+                            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb2, nullptr, 0))) break;
+                            if ((r = restore_frontier(g, O, os))) break;
+                            gpr_iter_stats st2 = {};
+                            if ((r = step(O, O2, c2, iter + 1, &st2))) break;
+                            t2 = std::min(t2, st2.ms);
+                        }
+                        t += t2;
+                    }
+                    tc = std::min(tc, t);
+                }
+                if (times) times[cnt * 4 + c] = (float)tc;
+                if (tc < best) {
+                    best = tc;
+                    bestc = c;
+                }
+            }
+            if (r) break;
+            if (features)
+                for (int i = 0; i < 6; ++i) features[cnt * 6 + i] = f[i];
+            ++cnt;
+            // advance with the fastest choice
+            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb, nullptr, 0))) break;
+            if ((r = restore_frontier(g, F, fs))) break;
+            if ((r = step(F, O, bestc, iter, nullptr))) break;
+            U -= (double)F->nacte;
+            std::swap(F, O);
+            ++iter;
+        }
+    } while (0);
+    free_copy(fs);
+    free_copy(os);
+    if (P) gpr_prop_destroy(P);
+    if (Pb) gpr_prop_destroy(Pb);
+    if (Pb2) gpr_prop_destroy(Pb2);
+    if (F) gpr_frontier_destroy(F);
+    if (O) gpr_frontier_destroy(O);
+    if (O2) gpr_frontier_destroy(O2);
+    g->pol_params = saved;
+    *k = cnt;
+    return r;
+}

@@ -286,6 +286,33 @@ int gpr_batch_download(const gpr_batch *b, uint64_t *index, uint32_t *nbrs,
 int gpr_batch_cc(gpr_batch *b, uint32_t *labels, gpr_run_info *info);
 int gpr_batch_destroy(gpr_batch *b);
 
+/* ---- learned direction / representation policy (GPR_DIR_POLICY) --------
+ * No counterpart in the reference, whose rule is the fixed threshold of
+ * api.h:150-153 (kept as the fallback when no network is loaded).  A
+ * 6-32-32-4 FP32 MLP picks {push, pull} x {list, bitmap} output each
+ * step; its forward pass runs in the last workgroup of the step's kernel.
+ * Parameter layout: W1[6][32] b1[32] W2[32][32] b2[32] W3[32][4] b3[4]. */
+int gpr_mlp_param_count(void);
+/* Load (params != NULL) or drop (NULL) the network of this graph handle. */
+int gpr_policy_set(gpr_graph *g, const float *params, int count);
+/* Start of a run: iteration 0, all edges unvisited, no pending choice. */
+int gpr_policy_reset(gpr_graph *g);
+/* Training traces: algo 0 = BFSLVLv from start, 1 = CCv.  For each of at
+ * most cap iterations: the 6 features and the measured ms of the 4 choices
+ * (each plus the better next step).  *k receives the iteration count. */
+int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
+                     float *features, float *times, int *k);
+/* The network on the GPU: logits of count samples; gradient of the
+ * weighted soft-target cross-entropy (sum_i w_i CE_i / sum_i w_i); and
+ * full-batch Adam training in place. */
+int gpr_mlp_forward(const float *params, const float *x, int count,
+                    float *logits);
+int gpr_mlp_grad(const float *params, const float *x, const float *targets,
+                 const float *weights, int count, float *grad, float *loss);
+int gpr_mlp_train(float *params, const float *x, const float *targets,
+                  const float *weights, int count, int epochs, float lr,
+                  float *loss);
+
 /* Per-iteration trace of the last run (-itimes, BFSLVLv.C:90-106). */
 int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
 

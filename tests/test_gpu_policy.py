@@ -1,0 +1,162 @@
+"""GPU tests of the learned direction/representation policy.
+
+  * the MLP forward pass and gradients computed by the HIP kernels equal a
+    float64 numpy restatement (the CPU oracle of the network), and a central
+    finite difference of the loss confirms that restatement;
+  * Adam training on the GPU lowers the loss;
+  * BFS and CC stay bit-exact whatever a (random) network chooses, with the
+    choice made by the fused in-kernel inference;
+  * traces record four timed choices per iteration.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+G = pytest.importorskip("graptor_amd")
+
+IN, H, OUT = 6, 32, 4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    G.lib()
+    yield
+
+
+def unpack(p):
+    p = np.asarray(p, np.float64)
+    o = 0
+    W1 = p[o:o + IN * H].reshape(IN, H); o += IN * H
+    b1 = p[o:o + H]; o += H
+    W2 = p[o:o + H * H].reshape(H, H); o += H * H
+    b2 = p[o:o + H]; o += H
+    W3 = p[o:o + H * OUT].reshape(H, OUT); o += H * OUT
+    b3 = p[o:o + OUT]; o += OUT
+    assert o == len(p)
+    return W1, b1, W2, b2, W3, b3
+
+
+def np_forward(p, X):
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    A1 = X @ W1 + b1
+    H1 = np.maximum(A1, 0)
+    A2 = H1 @ W2 + b2
+    H2 = np.maximum(A2, 0)
+    return H1, H2, H2 @ W3 + b3
+
+
+def np_loss_grad(p, X, T, w):
+    X = X.astype(np.float64)
+    H1, H2, Z = np_forward(p, X)
+    W1, b1, W2, b2, W3, b3 = unpack(p)
+    Zs = Z - Z.max(1, keepdims=True)
+    P = np.exp(Zs) / np.exp(Zs).sum(1, keepdims=True)
+    lse = np.log(np.exp(Zs).sum(1, keepdims=True)) + Z.max(1, keepdims=True)
+    ws = w.sum()
+    loss = float((w[:, None] * -T * (Z - lse)).sum() / ws)
+    dZ = w[:, None] * (P - T) / ws
+    gW3 = H2.T @ dZ
+    gb3 = dZ.sum(0)
+    dH2 = (dZ @ W3.T) * (H2 > 0)
+    gW2 = H1.T @ dH2
+    gb2 = dH2.sum(0)
+    dH1 = (dH2 @ W2.T) * (H1 > 0)
+    gW1 = X.T @ dH1
+    gb1 = dH1.sum(0)
+    return loss, np.concatenate([gW1.ravel(), gb1, gW2.ravel(), gb2, gW3.ravel(), gb3])
+
+
+def random_problem(n=300, seed=0):
+    rng = np.random.default_rng(seed)
+    k = G.mlp_param_count()
+    p = (rng.standard_normal(k) * 0.3).astype(np.float32)
+    X = rng.random((n, IN)).astype(np.float32)
+    T = rng.random((n, OUT)).astype(np.float32)
+    T /= T.sum(1, keepdims=True)
+    w = (rng.random(n) + 0.1).astype(np.float32)
+    return p, X, T, w
+
+
+def test_param_count():
+    assert G.mlp_param_count() == IN * H + H + H * H + H + H * OUT + OUT
+
+
+def test_forward_matches_numpy():
+    p, X, _, _ = random_problem()
+    Z = G.mlp_forward(p, X)
+    _, _, Zr = np_forward(p, X.astype(np.float64))
+    assert np.allclose(Z, Zr, rtol=1e-5, atol=1e-5)
+
+
+def test_gradient_matches_numpy_and_finite_differences():
+    p, X, T, w = random_problem(n=517)   # not a multiple of the workgroup size
+    g, loss = G.mlp_grad(p, X, T, w)
+    lr, gr = np_loss_grad(p.astype(np.float64), X, T.astype(np.float64), w.astype(np.float64))
+    assert abs(loss - lr) < 1e-4 * max(1.0, abs(lr))
+    assert np.allclose(g, gr, rtol=1e-3, atol=2e-6)
+    # the numpy restatement itself against central differences
+    rng = np.random.default_rng(1)
+    pd = p.astype(np.float64)
+    for i in rng.choice(len(p), 25, replace=False):
+        e = np.zeros_like(pd)
+        e[i] = 1e-6
+        fd = (np_loss_grad(pd + e, X, T.astype(np.float64), w.astype(np.float64))[0]
+              - np_loss_grad(pd - e, X, T.astype(np.float64), w.astype(np.float64))[0]) / 2e-6
+        assert abs(fd - gr[i]) < 1e-5 + 1e-3 * abs(gr[i])
+
+
+def test_training_lowers_loss():
+    p, X, T, w = random_problem(n=800, seed=3)
+    _, l0 = G.mlp_grad(p, X, T, w)
+    p2, l1 = G.mlp_train(p, X, T, w, epochs=300, lr=1e-2)
+    _, l2 = G.mlp_grad(p2, X, T, w)
+    assert l2 < l0 and abs(l1 - l2) < 0.05 * max(l2, 1e-3) + 1e-3
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_policy_keeps_results_bit_exact(seed):
+    p, _, _, _ = random_problem(seed=seed)
+    for name, (index, nbrs) in (("rmat14", O.rmat_graph(14)), ("grid", O.grid_graph(50, 40, 0.01, seed=2))):
+        s = O.highest_degree_vertex(index)
+        with G.Graph.from_csr(index, nbrs) as g:
+            g.set_policy(p)
+            lv, info = g.bfslvl(s, direction=G.DIR_POLICY, trace=True)
+            assert np.array_equal(lv, O.bfs_levels(index, nbrs, s)), name
+            tr = g.trace()
+            assert len(tr) == info.iters
+            lab, _ = g.ccv(direction=G.DIR_POLICY)
+            assert np.array_equal(lab, O.cc_labels(index, nbrs)), name
+            g.set_policy(None)     # back to the threshold rule
+            lv2, _ = g.bfslvl(s, direction=G.DIR_POLICY)
+            assert np.array_equal(lv2, lv)
+
+
+def test_policy_choice_follows_network():
+    """A network whose output bias forces one choice is obeyed each step."""
+    k = G.mlp_param_count()
+    for forced, (d, kind) in enumerate([(G.DIR_PUSH, G.FRONTIER_LIST), (G.DIR_PUSH, G.FRONTIER_BITMAP),
+                                        (G.DIR_PULL, G.FRONTIER_LIST), (G.DIR_PULL, G.FRONTIER_BITMAP)]):
+        p = np.zeros(k, np.float32)
+        p[k - OUT + forced] = 10.0
+        index, nbrs = O.rmat_graph(12)
+        with G.Graph.from_csr(index, nbrs) as g:
+            g.set_policy(p)
+            lv, _ = g.bfslvl(0, direction=G.DIR_POLICY, trace=True)
+            tr = g.trace()
+            assert all(t["direction"] == d and t["out_kind"] in (kind, G.FRONTIER_EMPTY) or t["nactv_out"] == 0
+                       for t in tr)
+            assert np.array_equal(lv, O.bfs_levels(index, nbrs, 0))
+
+
+def test_trace_records_four_choices():
+    index, nbrs = O.rmat_graph(12)
+    with G.Graph.from_csr(index, nbrs) as g:
+        f, t = g.policy_trace(0, O.highest_degree_vertex(index))
+        lv, info = g.bfslvl(O.highest_degree_vertex(index))
+        assert f.shape[0] == t.shape[0] >= 3 and f.shape[1] == 6 and t.shape[1] == 4
+        assert (t > 0).all() and np.isfinite(f).all()
+        assert abs(f.shape[0] - info.iters) <= 1
+        f2, t2 = g.policy_trace(1)
+        assert len(f2) >= 2
