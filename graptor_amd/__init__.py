@@ -127,6 +127,8 @@ SIGNATURES = {
     "gpr_mlp_grad": (C.c_int, [_P, _P, _P, _P, C.c_int, _P, C.POINTER(C.c_float)]),
     "gpr_mlp_train": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, C.c_float, C.POINTER(C.c_float)]),
     "gpr_trace_get": (C.c_int, [_P, C.POINTER(IterStats), C.c_int, C.POINTER(C.c_int)]),
+    "gpr_tune_set": (C.c_int, [C.c_char_p, C.c_double]),
+    "gpr_tune_get": (C.c_int, [C.c_char_p, C.POINTER(C.c_double)]),
     "gpr_profile_enable": (C.c_int, [_P, C.c_int]),
     "gpr_profile_get": (C.c_int, [_P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "gpr_profile_reset": (C.c_int, [_P]),
@@ -151,14 +153,33 @@ def load_library(path=LIB_PATH):
     return _lib
 
 
+TUNED = os.path.join(_HERE, "tuned.json")
+
+
 def lib():
-    """The library with the gfx950 device initialised."""
+    """The library with the gfx950 device initialised and the autotuned
+    parameters (graptor_amd/tuned.json, when present) applied."""
     global _inited
     L = load_library()
     if not _inited:
         _check(L.gpr_init(0), L)
         _inited = True
+        if os.path.exists(TUNED) and not os.environ.get("GPR_NO_TUNED"):
+            import json
+            with open(TUNED) as f:
+                for k, v in json.load(f).get("tunables", {}).items():
+                    _check(L.gpr_tune_set(k.encode(), float(v)), L)
     return L
+
+
+def tune_set(name, value):
+    _check(lib().gpr_tune_set(name.encode(), float(value)))
+
+
+def tune_get(name):
+    v = C.c_double()
+    _check(lib().gpr_tune_get(name.encode(), C.byref(v)))
+    return v.value
 
 
 def _check(rc, L=None):

@@ -1,0 +1,97 @@
+"""Autotuner of the engine's runtime parameters.
+
+Coordinate descent over the tunables of gpr_tune_set on a relabelled RMAT
+graph (the benchmark's graph family): each parameter in turn is swept over
+its candidates with the others fixed at their best so far, and the value
+with the lowest median time (over `reps` runs) of the kernels it affects
+is kept:
+
+  pr_hot_bytes        PR pull time, FP32 and BF16 contributions together
+  pull_serial_max     BFS + CC run time (the pull kernel's lane/wave split)
+  pull_blocks_per_cu  BFS + CC run time
+  push_blocks_per_cu  BFS + CC run time
+
+The chosen values and every measurement go to graptor_amd/tuned.json, which
+graptor_amd.lib() applies at start-up.  Run ``python -m graptor_amd.autotune
+[--scale 26]``.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import TUNED, Graph, tune_get, tune_set
+
+CANDIDATES = {
+    "pr_hot_bytes": [0, 1 << 20, 2 << 20, 4 << 20, 16 << 20, 1 << 36],
+    "pull_serial_max": [8, 16, 32, 64, 128],
+    "pull_blocks_per_cu": [4, 8, 16, 32],
+    "push_blocks_per_cu": [8, 32, 128],
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def median_ms(fn, reps):
+    fn()
+    return float(np.median([fn() for _ in range(reps)]))
+
+
+def measure(g, which, reps, src):
+    if which == "pr_hot_bytes":
+        def f():
+            _, i1 = g.pr_vary([("fp32", 0.0)], max_iter=3)
+            _, i2 = g.pr_vary([("bf16", 0.0)], max_iter=3)
+            return i1.ms + i2.ms
+        return median_ms(f, reps)
+
+    def f():
+        _, ib = g.bfslvl(src)
+        _, ic = g.ccv()
+        return ib.ms + ic.ms
+    return median_ms(f, reps)
+
+
+def tune(scale=26, reps=3, out=TUNED):
+    t0 = time.time()
+    g = Graph.rmat(scale)
+    g.reorder()
+    src = g.highest_degree_vertex()
+    log("autotune on RMAT-%d: n=%d m=%d" % (scale, g.n, g.m))
+    best = {k: tune_get(k) for k in CANDIDATES}
+    record = {}
+    for name, cands in CANDIDATES.items():
+        times = {}
+        for v in cands:
+            tune_set(name, v)
+            times[v] = measure(g, name, reps, src)
+            log("  %s = %g: %.3f ms" % (name, v, times[v]))
+        best[name] = min(times, key=times.get)
+        tune_set(name, best[name])
+        record[name] = {str(k): v for k, v in times.items()}
+        log("-> %s = %g" % (name, best[name]))
+    g.destroy()
+    res = {"tunables": best, "measurements_ms": record, "workload": "RMAT-%d relabelled, median of %d" % (scale, reps),
+           "seconds": time.time() - t0}
+    with open(out, "w") as f:
+        json.dump(res, f, indent=1)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scale", type=int, default=26)
+    ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--out", default=TUNED)
+    args = ap.parse_args()
+    os.environ["GPR_NO_TUNED"] = "1"   # start from the built-in defaults
+    print(json.dumps(tune(args.scale, args.reps, args.out)))
+
+
+if __name__ == "__main__":
+    main()

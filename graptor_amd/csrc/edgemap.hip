@@ -28,6 +28,8 @@
 #include "device.hpp"
 #include "policy_dev.hpp"
 
+#include <algorithm>
+
 namespace gpr {
 
 // ---- degree prefix of a frontier list -------------------------------------
@@ -108,23 +110,23 @@ template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) 
 __device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounters *ctr) {
     if (!pol.params) return;
     __shared__ bool last;
+    __shared__ PolicyLds lds;
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         last = atomicAdd(&ctr->ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
-    if (last && threadIdx.x < 64) {
-        __threadfence();
-        const uint32_t nv = __hip_atomic_load(&ctr->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint64_t ne = __hip_atomic_load(&ctr->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        float f[6];
-        policy_features(pol, nv, ne, f);
-        const int d = policy_forward_wave(pol.params, f);
-        if (threadIdx.x == 0) {
-            ctr->decision = (unsigned)d;
-            ctr->ticket = 0;
-        }
+    if (!last) return;   // uniform across the workgroup
+    __threadfence();
+    const uint32_t nv = __hip_atomic_load(&ctr->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t ne = __hip_atomic_load(&ctr->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    float f[6];
+    policy_features(pol, nv, ne, f);
+    const int d = policy_forward_block(pol.params, f, lds);
+    if (threadIdx.x == 0) {
+        ctr->decision = (unsigned)d;
+        ctr->ticket = 0;
     }
 }
 
@@ -239,7 +241,7 @@ __global__ void __launch_bounds__(PULL_THREADS)
 k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
        const uint32_t *__restrict__ deg, uint32_t *prop, const uint32_t *__restrict__ prev, int dkind,
        uint32_t darg, int record, const uint64_t *__restrict__ in_bits, uint64_t *out_bits,
-       uint32_t *out_list, DevCounters *ctr, PolicyCtx pol) {
+       uint32_t *out_list, DevCounters *ctr, PolicyCtx pol, uint32_t serial_max) {
     __shared__ uint64_t red[PULL_THREADS / 64];
     __shared__ uint32_t red32[PULL_THREADS / 64];
     const uint32_t ngroups = (n + 63) / 64;
@@ -259,7 +261,7 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
             beg = cindex[d];
             end = cindex[d + 1];
         }
-        const bool heavy = act && (end - beg) > PULL_SERIAL_MAX;
+        const bool heavy = act && (end - beg) > serial_max;
         if (act && !heavy) {
             // PULL_UNROLL independent neighbour loads (and frontier tests)
             // in flight per lane; edges past an early exit are loaded but
@@ -432,7 +434,7 @@ int scan_frontier_degrees(gpr_graph *g, const uint32_t *list, uint32_t k) {
 
 static unsigned pull_grid(uint32_t n) {
     uint32_t groups = (n + 63) / 64;
-    return grid_for(groups, PULL_THREADS / 64, 2048);
+    return grid_for(groups, PULL_THREADS / 64, (unsigned)std::max(1.0, tun().pull_blocks_per_cu * num_cus()));
 }
 
 int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st);
@@ -506,7 +508,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         }
         uint32_t *olist = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
         if (E) {
-            const unsigned grid = grid_for(E, PUSH_TILE, 8192);
+            const unsigned grid = grid_for(E, PUSH_TILE, (unsigned)std::max(1.0, tun().push_blocks_per_cu * num_cus()));
             if (desc->relax == GPR_RELAX_MIN_PLUS1)
                 hipLaunchKernelGGL(k_push<GPR_RELAX_MIN_PLUS1>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
                                    g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
@@ -526,11 +528,11 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         if (desc->relax == GPR_RELAX_MIN_PLUS1)
             hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN_PLUS1>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n,
                                g->cindex, g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits,
-                               ob, ol, g->ctr, pol);
+                               ob, ol, g->ctr, pol, (uint32_t)tun().pull_serial_max);
         else
             hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex,
                                g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol,
-                               g->ctr, pol);
+                               g->ctr, pol, (uint32_t)tun().pull_serial_max);
     }
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipEventRecord(g->ev1, g->stream));

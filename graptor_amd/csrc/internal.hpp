@@ -59,6 +59,16 @@ void set_error(const char *fmt, ...);
 int dev_alloc(void **p, size_t bytes, const char *what);
 void dev_free(void *p);
 
+// ---- runtime tunables (set by the autotuner through gpr_tune_set) --------
+struct Tunables {
+    double pr_hot_bytes = 2.0 * 1024 * 1024;  // contributions gathered through L2
+    double pull_serial_max = 32;              // lane-serial rows up to this degree
+    double pull_blocks_per_cu = 8;            // pull grid = CUs * this
+    double push_blocks_per_cu = 32;           // push grid cap = CUs * this
+};
+Tunables &tun();
+int num_cus();
+
 // ---- device-resident counters of one handle ------------------------------
 struct DevCounters {
     unsigned long long nacte;   // edges of the frontier being produced
@@ -136,6 +146,7 @@ struct gpr_graph {
     std::vector<void *> pool_list, pool_bits;
     // learned policy state of the current run
     float *pol_params = nullptr;    // device copy of the network (null: none)
+    std::vector<float> pol_host;    // host copy (first-step decision)
     bool pol_have = false;          // a device decision for the next step exists
     int pol_next = 0;
     int pol_iter = 0;

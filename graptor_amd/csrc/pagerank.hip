@@ -87,6 +87,13 @@ constexpr int PR_LDS_EDGES = STREAM_ITEMS + (int)LONG_DEG;
 constexpr uint32_t PR_MED = 64;
 constexpr int PR_UNROLL = PR_LDS_EDGES / PR_THREADS;  // 16
 
+// Contributions of the first `hot` (highest-degree) vertices are gathered
+// through L2; the rest with non-temporal loads, so that rarely reused lines
+// do not evict the hot ones.
+template <class T> __device__ __forceinline__ T gather(const T *__restrict__ c, uint32_t s, uint32_t hot) {
+    return s < hot ? c[s] : __builtin_nontemporal_load(&c[s]);
+}
+
 template <int FMT>
 __global__ void __launch_bounds__(PR_THREADS)
 k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t *__restrict__ cindex,
@@ -94,7 +101,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
           const typename Codec<FMT>::store_t *__restrict__ contrib, float *__restrict__ y,
           const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ long_base, uint32_t nlong,
           double *__restrict__ long_partial, double *__restrict__ part, float *__restrict__ part_max,
-          const PrScalars *__restrict__ prs) {
+          const PrScalars *__restrict__ prs, uint32_t hot) {
     using C = Codec<FMT>;
     using acc_t = typename C::acc_t;
     __shared__ float vals[PR_LDS_EDGES];
@@ -130,7 +137,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
             }
             float v[PR_UNROLL];
 #pragma unroll
-            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(contrib[s[u]]) : 0.0f;
+            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(gather(contrib, s[u], hot)) : 0.0f;
 #pragma unroll
             for (int u = 0; u < PR_UNROLL; ++u) acc += (acc_t)v[u];
         }
@@ -177,7 +184,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
             }
             float v[PR_UNROLL];
 #pragma unroll
-            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(contrib[s[u]]) : 0.0f;
+            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(gather(contrib, s[u], hot)) : 0.0f;
 #pragma unroll
             for (int u = 0; u < PR_UNROLL; ++u) {
                 const uint32_t i = base + u * PR_THREADS + threadIdx.x;
@@ -196,8 +203,10 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
             const float yv = (float)acc * dscale;
             y[r0 + i] = yv;
             my_sum += (double)yv;
-            const uint32_t dg = deg[r0 + i];
-            if (dg) my_max = fmaxf(my_max, yv / (float)dg);
+            if (FMT == GPR_CONTRIB_F16) {
+                const uint32_t dg = deg[r0 + i];
+                if (dg) my_max = fmaxf(my_max, yv / (float)dg);
+            }
         }
         __syncthreads();
         const uint32_t nm = nmed;
@@ -212,8 +221,10 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
                 const float yv = (float)acc * dscale;
                 y[r0 + i] = yv;
                 my_sum += (double)yv;
-                const uint32_t dg = deg[r0 + i];
-                if (dg) my_max = fmaxf(my_max, yv / (float)dg);
+                if (FMT == GPR_CONTRIB_F16) {
+                    const uint32_t dg = deg[r0 + i];
+                    if (dg) my_max = fmaxf(my_max, yv / (float)dg);
+                }
             }
         }
     }
@@ -406,13 +417,15 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
     GPR_TRY(ensure_partials(g, (size_t)nblk + g->nlong + 4096));
     float *y = (float *)desc->prop->d;
     const void *cb = desc->contrib->d;
+    const double hb = tun().pr_hot_bytes / (double)fmt_bytes(fmt);
+    const uint32_t hot = hb >= (double)g->n ? g->n : (uint32_t)hb;
     GPR_HIP(hipEventRecord(g->ev0, g->stream));
     {
         ProfScope ps(g, "pr_pull");
 #define PR_PULL(F)                                                                                              \
     hipLaunchKernelGGL(k_pr_pull<F>, dim3(nblk), dim3(PR_THREADS), 0, g->stream, g->nbuckets, g->bucket,        \
                        g->cindex, g->cnbrs, g->deg, (const typename Codec<F>::store_t *)cb, y, g->long_rows,   \
-                       g->long_base, g->nlong, g->long_partial, g->partial, g->partial_f, g->prs)
+                       g->long_base, g->nlong, g->long_partial, g->partial, g->partial_f, g->prs, hot)
         switch (fmt) {
         case GPR_CONTRIB_F32: PR_PULL(GPR_CONTRIB_F32); break;
         case GPR_CONTRIB_BF16: PR_PULL(GPR_CONTRIB_BF16); break;

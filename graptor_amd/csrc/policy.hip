@@ -86,11 +86,9 @@ int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *o
     if (g->pol_have) {
         d = g->pol_next;
     } else {
-        std::vector<float> p(MLP_PARAMS);
-        GPR_HIP(hipMemcpy(p.data(), g->pol_params, sizeof(float) * MLP_PARAMS, hipMemcpyDeviceToHost));
         float f[6];
         host_features(g, nactv, nacte, g->pol_unvisited - (double)nacte, g->pol_iter, f);
-        d = host_forward(p.data(), f);
+        d = host_forward(g->pol_host.data(), f);
     }
     *dir = d < 2 ? GPR_DIR_PUSH : GPR_DIR_PULL;
     *out_kind = (d & 1) ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST;
@@ -326,12 +324,14 @@ extern "C" int gpr_policy_set(gpr_graph *g, const float *params, int count) {
     if (!params) {
         dev_free(g->pol_params);
         g->pol_params = nullptr;
+        g->pol_host.clear();
         return GPR_OK;
     }
     GPR_REQUIRE(count == MLP_PARAMS, "policy: expected %d parameters, got %d", MLP_PARAMS, count);
     for (int i = 0; i < count; ++i) GPR_REQUIRE(std::isfinite(params[i]), "policy: parameter %d is not finite", i);
     if (!g->pol_params) GPR_TRY(dev_alloc((void **)&g->pol_params, 4ull * MLP_PARAMS, "policy"));
     GPR_HIP(hipMemcpy(g->pol_params, params, 4ull * MLP_PARAMS, hipMemcpyHostToDevice));
+    g->pol_host.assign(params, params + MLP_PARAMS);
     return GPR_OK;
 }
 

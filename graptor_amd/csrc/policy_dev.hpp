@@ -49,6 +49,46 @@ __device__ __forceinline__ void policy_features(const PolicyCtx &c, uint32_t nac
     f[5] = c.skew_feat;
 }
 
+// The whole (last) workgroup evaluates the network: the parameters are
+// staged in LDS with one load round, thread j computes hidden unit j.
+// Every thread of the workgroup must call it; returns the argmax.
+struct PolicyLds {
+    float p[MLP_PARAMS];
+    float h[2 * MLP_H];
+    float z[MLP_OUT];
+};
+__device__ __forceinline__ int policy_forward_block(const float *params, const float f[6], PolicyLds &L) {
+    for (int i = threadIdx.x; i < MLP_PARAMS; i += blockDim.x) L.p[i] = params[i];
+    __syncthreads();
+    const MlpView P(L.p);
+    const int j = threadIdx.x;
+    if (j < MLP_H) {
+        float a = P.b1[j];
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) a += f[i] * P.w1[i * MLP_H + j];
+        L.h[j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < MLP_H) {
+        float a = P.b2[j];
+#pragma unroll 8
+        for (int i = 0; i < MLP_H; ++i) a += L.h[i] * P.w2[i * MLP_H + j];
+        L.h[MLP_H + j] = a > 0.f ? a : 0.f;
+    }
+    __syncthreads();
+    if (j < MLP_OUT) {
+        float a = P.b3[j];
+#pragma unroll 8
+        for (int i = 0; i < MLP_H; ++i) a += L.h[MLP_H + i] * P.w3[i * MLP_OUT + j];
+        L.z[j] = a;
+    }
+    __syncthreads();
+    int best = 0;
+    for (int k = 1; k < MLP_OUT; ++k)
+        if (L.z[k] > L.z[best]) best = k;
+    return best;
+}
+
 // One wave (64 lanes, all active) evaluates the network with lane j holding
 // hidden unit j (activations move between lanes by shuffles); returns the
 // argmax in every lane.

@@ -114,6 +114,22 @@ int deterministic_sum(gpr_graph *g, const double *parts, uint32_t count, double 
     return GPR_OK;
 }
 
+Tunables &tun() {
+    static Tunables t;
+    return t;
+}
+
+int num_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return cus;
+}
+
 std::vector<gpr_iter_stats> &trace_of(gpr_graph *g) { return g->trace; }
 void trace_clear(gpr_graph *g) { g->trace.clear(); }
 
@@ -176,6 +192,39 @@ extern "C" int gpr_init(int device) {
         return GPR_E_NODEV;
     }
     GPR_HIP(hipSetDevice(device));
+    return GPR_OK;
+}
+
+static double *tunable(const char *name) {
+    Tunables &t = tun();
+    if (!strcmp(name, "pr_hot_bytes")) return &t.pr_hot_bytes;
+    if (!strcmp(name, "pull_serial_max")) return &t.pull_serial_max;
+    if (!strcmp(name, "pull_blocks_per_cu")) return &t.pull_blocks_per_cu;
+    if (!strcmp(name, "push_blocks_per_cu")) return &t.push_blocks_per_cu;
+    return nullptr;
+}
+
+extern "C" int gpr_tune_set(const char *name, double value) {
+    double *p = name ? tunable(name) : nullptr;
+    if (!p) {
+        set_error("unknown tunable '%s'", name ? name : "(null)");
+        return GPR_E_INVAL;
+    }
+    if (!(value >= 0) || value > 1e12) {
+        set_error("tunable %s: value %g out of range", name, value);
+        return GPR_E_INVAL;
+    }
+    *p = value;
+    return GPR_OK;
+}
+
+extern "C" int gpr_tune_get(const char *name, double *value) {
+    double *p = name ? tunable(name) : nullptr;
+    if (!p || !value) {
+        set_error("unknown tunable '%s'", name ? name : "(null)");
+        return GPR_E_INVAL;
+    }
+    *value = *p;
     return GPR_OK;
 }
 

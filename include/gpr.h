@@ -316,6 +316,16 @@ int gpr_mlp_train(float *params, const float *x, const float *targets,
 /* Per-iteration trace of the last run (-itimes, BFSLVLv.C:90-106). */
 int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
 
+/* Process-wide runtime tunables (no reference counterpart; chosen by the
+ * autotuner, graptor_amd/autotune.py):
+ *   pr_hot_bytes        contribution bytes gathered through L2 (the rest of
+ *                       the gathers use non-temporal loads)
+ *   pull_serial_max     pull rows up to this in-degree run lane-serially
+ *   pull_blocks_per_cu  pull grid size per compute unit
+ *   push_blocks_per_cu  push grid cap per compute unit */
+int gpr_tune_set(const char *name, double value);
+int gpr_tune_get(const char *name, double *value);
+
 /* Device-event timing of named kernels over the runs since the last reset,
  * measured on the handle's own stream.  name: "pr_pull", "pr_update",
  * "push", "pull", ... */
