@@ -16,6 +16,9 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpr.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+DRIVER_SRC = os.path.join(HERE, "driver", "gpr_driver.cpp")
+DRIVER = os.path.join(HERE, "gpr_driver")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -70,7 +73,21 @@ def build(force=True, verbose=False, jobs=None):
         if r.returncode != 0:
             raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
         os.replace(LIB + ".tmp", LIB)
+    if todo or not os.path.exists(DRIVER) or os.path.getmtime(DRIVER) < os.path.getmtime(DRIVER_SRC):
+        build_driver()
     return LIB
+
+
+def build_driver():
+    """Link the command-line harness graptor_amd/gpr_driver against libgpr.so
+    (host code only, so the system C++ compiler builds it)."""
+    cmd = [CXX, "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L" + HERE, "-lgpr", "-Wl,-rpath,$ORIGIN", "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

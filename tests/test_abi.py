@@ -121,3 +121,27 @@ void bfs(gpr_graph *g, uint32_t start) {
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.join(ROOT, "include"),
                         "-x", "c++", "-"], input=src, text=True, capture_output=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_driver_command_line_without_device():
+    """graptor_amd/gpr_driver (the bench/driver.C shape): usage, flag errors
+    and the missing-device error are reported with exit codes, not aborts.
+    Nothing here initialises a device."""
+    import graptor_amd as G
+    from graptor_amd import build as B
+    G.load_library()
+    if not os.path.exists(B.DRIVER):
+        B.build_driver()
+    r = subprocess.run([B.DRIVER, "-h"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "-pr:maxiter" in r.stderr and "-cc:outfile" in r.stderr
+    r = subprocess.run([B.DRIVER, "-app", "sssp", "x.b2"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([B.DRIVER, "-app", "bfslvl", "-rounds"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "needs a value" in r.stderr
+    r = subprocess.run([B.DRIVER, "-app", "pr", "-dir", "sideways", "x.b2"], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 2 and "unknown -dir" in r.stderr
+    if not os.path.exists("/dev/kfd"):
+        r = subprocess.run([B.DRIVER, "-app", "bfslvl", "-b", "-s", "x.b2"], capture_output=True, text=True,
+                           timeout=60)
+        assert r.returncode == 1 and "gpr_init" in r.stderr
