@@ -7,8 +7,9 @@ one PRv run of 20 FP32 power iterations, on a symmetric RMAT graph of scale
 GPU before timing.  `value` is whole-job GTEPS: (BFS edges of the reached
 component + m for CC + 20*m for PR) / wall time of the K timed steps.
 
-Also reported: per-algorithm GTEPS, the HBM roofline of the dominant kernel
-(the PR pull, timed with HIP events on the engine's own stream), BF16/FP16
+Also reported: per-algorithm GTEPS, the HBM roofline of the dominant step
+(the PR edgemap: its kernels timed with HIP events on the engine's own
+stream), BF16/FP16
 PR per-iteration GTEPS, and the CPU baseline (the OpenMP restatement of
 Graptor's algorithm in oracle/, timed on the host on a bounded sample).
 """
@@ -31,15 +32,38 @@ def log(*a):
 
 
 def pr_pull_bytes(n, m, s):
-    """Algorithmic bytes of one PR pull launch: CSC offsets 8(n+1), neighbour
-    IDs 4m, one contribution gather of s bytes per edge, the y write 4n."""
+    """Algorithmic bytes of one PR edgemap y = A^T contrib: CSC offsets
+    8(n+1), neighbour IDs 4m, one contribution of s bytes per edge, the y
+    write 4n."""
     return 8 * (n + 1) + 4 * m + s * m + 4 * n
+
+
+# The kernels of one PR edgemap, by profile scope: the cold-source write
+# (with its mass reduction), the pull over the pulled prefix, the long-row
+# fold and the cold-source accumulation.  Without the split only the pull
+# and the long-row fold run.
+PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
+
+
+def pr_edgemap_ms(g):
+    """Per-scope average ms of the profiled PR edgemaps and their sum per
+    edgemap (each scope runs at most once per edgemap)."""
+    per = {}
+    launches = 0
+    for sc in PR_SCOPES:
+        ms, k = g.profile_get(sc)
+        if k:
+            per[sc] = ms / k
+        launches = max(launches, k)
+    return sum(per.values()), per, launches
 
 
 def run(args):
     import numpy as np
     import graptor_amd as G
 
+    if args.pr_split_bytes is not None:
+        G.tune_set("pr_split_bytes", args.pr_split_bytes)
     t0 = time.time()
     g = G.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, SEED)
     n, m = g.n, g.m
@@ -77,13 +101,13 @@ def run(args):
         log("step %d: bfs %.2f ms (%d iters), cc %.2f ms (%d iters), pr %.2f ms" % (k, ib.ms, ib.iters, ic.ms, ic.iters,
                                                                                  ip.ms))
     wall = time.perf_counter() - t_start
-    pull_ms, pull_launches = g.profile_get("pr_pull")
+    avg_pull_ms, per_scope, pull_launches = pr_edgemap_ms(g)
     g.profile(False)
     total_edges = eb + ec + ep
     value = total_edges / wall / 1e9
-    avg_pull_ms = pull_ms / max(1, pull_launches)
     bytes_pull = pr_pull_bytes(n, m, 4)
     achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
+    split = G.tune_get("pr_split_bytes")
     gteps = {
         "bfs": eb / (tb * 1e-3) / 1e9,
         "cc": ec / (tc * 1e-3) / 1e9,
@@ -112,7 +136,8 @@ def run(args):
         "gteps": gteps,
         "device_ms": {"bfs": tb / args.steps, "cc": tc / args.steps, "pr": tp / args.steps},
         "roofline": {
-            "kernel": "k_pr_pull<fp32>",
+            "kernel": ("PR edgemap <fp32>: k_cold_write + k_pr_pull + k_pr_long + k_cold_accum" if "pr_cold_accum" in
+                       per_scope else "PR edgemap <fp32>: k_pr_pull + k_pr_long"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -121,7 +146,9 @@ def run(args):
             "traffic": None,
             "algorithmic_bytes_per_launch": bytes_pull,
             "avg_launch_ms": avg_pull_ms,
+            "kernels_avg_ms": per_scope,
             "launches": pull_launches,
+            "pr_split_bytes": split,
         },
     }
     # reduced-precision PR, per iteration (PR_vary contribution formats)
@@ -132,11 +159,11 @@ def run(args):
             g.profile_reset()
             g.pr_vary([(fmt, 0.0)], max_iter=args.pr_iters)
             xr, info = g.pr_vary([(fmt, 0.0)], max_iter=args.pr_iters)
-            pm, pl = g.profile_get("pr_pull")
+            pm, _, _ = pr_edgemap_ms(g)
             g.profile(False)
             prec[fmt] = {"gteps_iter": info.edges / (info.ms * 1e-3) / 1e9,
-                         "pull_avg_ms": pm / max(1, pl),
-                         "pull_GBs": pr_pull_bytes(n, m, s) / (pm / max(1, pl) * 1e-3) / 1e9,
+                         "edgemap_avg_ms": pm,
+                         "edgemap_GBs": pr_pull_bytes(n, m, s) / (pm * 1e-3) / 1e9,
                          "speedup_vs_fp32_iter": (info.edges / info.ms) / (ep / tp)}
         res["pr_precision"] = prec
     if not args.no_cpu:
@@ -221,6 +248,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--pr-split-bytes", type=float, default=None,
+                    help="override the tunable pr_split_bytes (FP32 bytes of pulled sources; 0 = no split)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in the "

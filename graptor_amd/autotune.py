@@ -6,7 +6,7 @@ its candidates with the others fixed at their best so far, and the value
 with the lowest median time (over `reps` runs) of the kernels it affects
 is kept:
 
-  pr_hot_bytes        PR pull time, FP32 and BF16 contributions together
+  pr_split_bytes      PR edgemap time, FP32 and BF16 contributions together
   pull_serial_max     BFS + CC run time (the pull kernel's lane/wave split)
   pull_blocks_per_cu  BFS + CC run time
   push_blocks_per_cu  BFS + CC run time
@@ -26,7 +26,7 @@ import numpy as np
 from . import TUNED, Graph, tune_get, tune_set
 
 CANDIDATES = {
-    "pr_hot_bytes": [0, 1 << 20, 2 << 20, 4 << 20, 16 << 20, 1 << 36],
+    "pr_split_bytes": [0, 64 << 10, 256 << 10, 1 << 20, 2 << 20, 4 << 20, 8 << 20, 1e12],
     "pull_serial_max": [8, 16, 32, 64, 128],
     "pull_blocks_per_cu": [4, 8, 16, 32],
     "push_blocks_per_cu": [8, 32, 128],
@@ -43,7 +43,7 @@ def median_ms(fn, reps):
 
 
 def measure(g, which, reps, src):
-    if which == "pr_hot_bytes":
+    if which == "pr_split_bytes":
         def f():
             _, i1 = g.pr_vary([("fp32", 0.0)], max_iter=3)
             _, i2 = g.pr_vary([("bf16", 0.0)], max_iter=3)

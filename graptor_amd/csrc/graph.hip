@@ -134,6 +134,7 @@ static void graph_free(gpr_graph *g) {
     dev_free(g->dsum_slices);
     dev_free(g->prs);
     dev_free(g->pol_params);
+    cold_free(g);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->rev0) hipEventDestroy(g->rev0);
@@ -209,6 +210,7 @@ int graph_replace_csr(gpr_graph *g, uint64_t *index, uint32_t *nbrs) {
     dev_free(g->long_rows);
     dev_free(g->long_base);
     dev_free(g->long_partial);
+    cold_free(g);
     g->bucket = nullptr;
     g->long_rows = g->long_base = nullptr;
     g->long_partial = nullptr;

@@ -61,10 +61,10 @@ void dev_free(void *p);
 
 // ---- runtime tunables (set by the autotuner through gpr_tune_set) --------
 struct Tunables {
-    double pr_hot_bytes = 2.0 * 1024 * 1024;  // contributions gathered through L2
     double pull_serial_max = 32;              // lane-serial rows up to this degree
     double pull_blocks_per_cu = 8;            // pull grid = CUs * this
     double push_blocks_per_cu = 32;           // push grid cap = CUs * this
+    double pr_split_bytes = 0;                // FP32 bytes of pulled sources (0: all binned; >= 4n: no split)
 };
 Tunables &tun();
 int num_cus();
@@ -93,6 +93,40 @@ struct PrScalars {
     float enc_scale;   // contribution encode scale 2^k (FP16 only)
     float dec_scale;   // decode scale 2^-k of the stored contributions
     float pad;
+    double cold_unit;  // fixed-point unit of the binned (cold-source) sums
+};
+
+// Static propagation-blocking layout of the PageRank edges whose source is
+// at or above `from`: the sources outside the cache-resident prefix of a
+// degree-ordered graph, whose gathers would each cost a cache line (all
+// sources when from = 0).  Cold edges are grouped into tiles of consecutive
+// sources (<= COLD_TILE_V sources, <= COLD_TILE_E edges) and bins of
+// COLD_BIN destinations.  The write kernel reads each tile's entries in
+// (bin, destination) order and writes each value to its slot in the
+// bin-major value array, where a bin's entries are contiguous (its runs, one
+// per tile, each sorted by destination); the accumulation kernel streams
+// each bin into LDS.  Bins with more than COLD_CHUNK_E entries are split into
+// chunks whose integer partials a third kernel adds up.
+constexpr uint32_t COLD_BIN = 16384;
+constexpr uint32_t COLD_TILE_V = 32768;
+constexpr uint64_t COLD_TILE_E = 262144;
+constexpr uint64_t COLD_CHUNK_E = 1u << 20;
+constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
+struct ColdLayout {
+    uint32_t from = 0;              // first cold source
+    uint32_t ntiles = 0, nbins = 0;
+    uint64_t ncold = 0;             // cold edges (entries)
+    uint32_t *tile_lo = nullptr;    // ntiles+1 source boundaries
+    uint32_t *delta = nullptr;      // ntiles x nbins: bin-major slot - tile-major position of run (t,b)
+    uint32_t *wsrc = nullptr;       // per entry, tile-major: bin << 15 | (source - tile_lo[t])
+    uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN
+    uint32_t *vals = nullptr;       // per entry, bin-major: the staged contribution (4 or 2 bytes)
+    uint32_t nwchunks = 0, nachunks = 0, nmulti = 0;
+    uint32_t *wchunk = nullptr;     // per write chunk: tile, first entry, end entry (tile-major), 0
+    uint32_t *achunk = nullptr;     // per accumulation chunk: bin, first entry, end entry (bin-major), slot
+    uint32_t *multi = nullptr;      // per bin in several chunks: bin, first slot, chunks, 0
+    long long *partials = nullptr;  // COLD_BIN fixed-point sums per slot
+    double *mass = nullptr;         // per write chunk: sum of |contribution| over its entries
 };
 
 struct Profile {
@@ -152,6 +186,7 @@ struct gpr_graph {
     int pol_iter = 0;
     double pol_unvisited = 0;
     uint32_t max_deg = 0;
+    gpr::ColdLayout *cold = nullptr;   // PageRank cold-edge layout (built on first use)
 };
 
 struct gpr_prop {
@@ -199,6 +234,13 @@ struct ProfScope {
     }
 };
 void prof_flush(gpr_graph *g);
+
+// PageRank cold-edge layout (pr_cold.hip)
+int cold_prepare(gpr_graph *g, uint32_t from);   // (re)build for this split point
+void cold_free(gpr_graph *g);
+int cold_launch_write(gpr_graph *g, int fmt, const void *contrib);
+// y (+)= the binned sums; *nparts receives the number of y partial slots
+int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, float *part_max, uint32_t *nparts);
 
 // Host entry points shared between translation units.
 int frontier_ensure_list(gpr_frontier *f);

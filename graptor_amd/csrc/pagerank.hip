@@ -23,75 +23,29 @@
 // accumulated in FP32.
 #include "internal.hpp"
 #include "device.hpp"
+#include "pr_codec.hpp"
 
 #include <math.h>
 
 namespace gpr {
-
-// ---- contribution codecs ---------------------------------------------------
-template <int FMT> struct Codec;
-template <> struct Codec<GPR_CONTRIB_F32> {
-    using store_t = float;
-    using acc_t = double;
-    __device__ static float dec(float v) { return v; }
-    __device__ static float enc(float v) { return v; }
-};
-// bfloat16, round to nearest even
-template <> struct Codec<GPR_CONTRIB_BF16> {
-    using store_t = uint16_t;
-    using acc_t = float;
-    __device__ static float dec(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-    __device__ static uint16_t enc(float f) {
-        uint32_t u = __float_as_uint(f);
-        u += 0x7fffu + ((u >> 16) & 1u);
-        return (uint16_t)(u >> 16);
-    }
-};
-// scustomfp<true,8,7,false,0>: the upper half of the float, truncated
-// (customfp.h:95-117 with S=true, E=8, M=7, B=0)
-template <> struct Codec<GPR_CONTRIB_BF16T> {
-    using store_t = uint16_t;
-    using acc_t = float;
-    __device__ static float dec(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-    __device__ static uint16_t enc(float f) { return (uint16_t)(__float_as_uint(f) >> 16); }
-};
-// IEEE half (v_cvt_f16_f32, round to nearest even), values pre-scaled by
-// 2^k so that the largest contribution sits near the top of the range
-template <> struct Codec<GPR_CONTRIB_F16> {
-    using store_t = uint16_t;
-    using acc_t = float;
-    __device__ static float dec(uint16_t h) {
-        _Float16 x;
-        __builtin_memcpy(&x, &h, 2);
-        return (float)x;
-    }
-    __device__ static uint16_t enc(float f) {
-        _Float16 x = (_Float16)f;
-        uint16_t h;
-        __builtin_memcpy(&h, &x, 2);
-        return h;
-    }
-};
-// customfp<6,10>: no sign, the low 6 exponent bits and top 10 mantissa bits,
-// truncated; decode restores exponent bits 01 above them, so the format
-// covers [2^-63, 2) (customfp.h:95-117 encode, :141-169 decode)
-template <> struct Codec<GPR_CONTRIB_CFP16> {
-    using store_t = uint16_t;
-    using acc_t = float;
-    __device__ static float dec(uint16_t h) { return __uint_as_float(((uint32_t)h | 0x10000u) << 13); }
-    __device__ static uint16_t enc(float f) { return (uint16_t)(__float_as_uint(f) >> 13); }
-};
 
 constexpr int PR_THREADS = 256;
 constexpr int PR_LDS_EDGES = STREAM_ITEMS + (int)LONG_DEG;
 constexpr uint32_t PR_MED = 64;
 constexpr int PR_UNROLL = PR_LDS_EDGES / PR_THREADS;  // 16
 
-// Contributions of the first `hot` (highest-degree) vertices are gathered
-// through L2; the rest with non-temporal loads, so that rarely reused lines
-// do not evict the hot ones.
-template <class T> __device__ __forceinline__ T gather(const T *__restrict__ c, uint32_t s, uint32_t hot) {
-    return s < hot ? c[s] : __builtin_nontemporal_load(&c[s]);
+// PR_UNROLL gathers of one lane, issued back to back: every load is
+// unconditional (masked lanes read contrib[0]) and decoding follows all of
+// them, so no load waits for the previous one.  Sources at or above
+// cold_from (and the 0xffffffff padding) contribute 0.
+template <int FMT>
+__device__ __forceinline__ void gather_all(const typename Codec<FMT>::store_t *__restrict__ c, const uint32_t *s,
+                                           uint32_t cold_from, float *v) {
+    typename Codec<FMT>::store_t raw[PR_UNROLL];
+#pragma unroll
+    for (int u = 0; u < PR_UNROLL; ++u) raw[u] = c[s[u] < cold_from ? s[u] : 0u];
+#pragma unroll
+    for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] < cold_from ? Codec<FMT>::dec(raw[u]) : 0.0f;
 }
 
 template <int FMT>
@@ -101,7 +55,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
           const typename Codec<FMT>::store_t *__restrict__ contrib, float *__restrict__ y,
           const uint32_t *__restrict__ long_rows, const uint32_t *__restrict__ long_base, uint32_t nlong,
           double *__restrict__ long_partial, double *__restrict__ part, float *__restrict__ part_max,
-          const PrScalars *__restrict__ prs, uint32_t hot) {
+          const PrScalars *__restrict__ prs, uint32_t cold_from) {
     using C = Codec<FMT>;
     using acc_t = typename C::acc_t;
     __shared__ float vals[PR_LDS_EDGES];
@@ -136,8 +90,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
                 s[u] = e < ce ? __builtin_nontemporal_load(&cnbrs[e]) : 0xffffffffu;
             }
             float v[PR_UNROLL];
-#pragma unroll
-            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(gather(contrib, s[u], hot)) : 0.0f;
+            gather_all<FMT>(contrib, s, cold_from, v);
 #pragma unroll
             for (int u = 0; u < PR_UNROLL; ++u) acc += (acc_t)v[u];
         }
@@ -183,8 +136,7 @@ k_pr_pull(uint32_t nbuckets, const uint32_t *__restrict__ bucket, const uint64_t
                 s[u] = i < ne ? __builtin_nontemporal_load(&cnbrs[e0 + i]) : 0xffffffffu;
             }
             float v[PR_UNROLL];
-#pragma unroll
-            for (int u = 0; u < PR_UNROLL; ++u) v[u] = s[u] != 0xffffffffu ? C::dec(gather(contrib, s[u], hot)) : 0.0f;
+            gather_all<FMT>(contrib, s, cold_from, v);
 #pragma unroll
             for (int u = 0; u < PR_UNROLL; ++u) {
                 const uint32_t i = base + u * PR_THREADS + threadIdx.x;
@@ -260,7 +212,8 @@ __global__ void k_pr_long(uint32_t nlong, const uint32_t *__restrict__ long_rows
 
 // Deterministic reduction of `count` partial sums (and maxima).  mode 0:
 // after the pull -- set sum_y, w, inv_s and the encode scale.  mode 1: after
-// the update -- set delta and sum_x from the two partial arrays.
+// the update -- set delta and sum_x from the two partial arrays.  mode 2:
+// maximum only.  mode 3: the cold-edge mass -- set the fixed-point unit.
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 64;
 __global__ void __launch_bounds__(RED_THREADS)
@@ -276,7 +229,7 @@ k_pr_reduce(const double *__restrict__ part, const float *__restrict__ part_max,
     float mx = 0.0f;
     for (uint32_t i = lo + threadIdx.x; i < hi; i += RED_THREADS) {
         s1 += part[i];
-        if (mode != 1) mx = fmaxf(mx, part_max[i]);
+        if (mode == 0 || mode == 2) mx = fmaxf(mx, part_max[i]);
         if (mode == 1) s2 += part2[i];
     }
     r1[threadIdx.x] = s1;
@@ -329,6 +282,12 @@ k_pr_reduce(const double *__restrict__ part, const float *__restrict__ part_max,
     } else if (mode == 1) {
         prs->delta = a;
         prs->sum_x = c;
+    } else if (mode == 3) {
+        // fixed-point unit of the cold sums: 2^E with M <= 2^61 units, M the
+        // total |contribution| over the cold edges
+        double unit = 1.0;
+        if (a > 0.0 && a < 1e300) unit = ldexp(1.0, ilogb(a) + 1 - 61);
+        prs->cold_unit = unit;
     }
     // mode 2 (maximum only): the slices are read by k_pr_set_scale
     *ticket = 0;
@@ -406,6 +365,15 @@ static size_t fmt_bytes(int fmt) { return fmt == GPR_CONTRIB_F32 ? 4 : 2; }
 
 static unsigned update_grid(uint32_t n) { return grid_for(n, 256, 4096); }
 
+// First cold source of the PageRank edgemap: sources at or above it are
+// summed through the binned layout (pr_cold.hip), those below by the pull.
+// The split is the FP32 bytes of pulled contributions (tunable
+// pr_split_bytes): 0 bins every edge, >= 4n pulls every edge.
+static uint32_t split_point(gpr_graph *g) {
+    const double h = floor(tun().pr_split_bytes / 4.0);
+    return h >= (double)g->n ? g->n : (uint32_t)h;
+}
+
 int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st) {
     GPR_REQUIRE(!in || in->kind == GPR_FRONTIER_ALL, "PR edgemap runs over the all-true frontier only");
     GPR_REQUIRE(desc->prop && desc->prop->dtype == GPR_F32, "PR edgemap: accumulator must be an f32 property");
@@ -413,19 +381,34 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
     GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_BF16T, "PR edgemap: unknown contribution format %d", fmt);
     GPR_REQUIRE(desc->contrib && desc->contrib->elem == fmt_bytes(fmt),
                 "PR edgemap: contribution property has the wrong element size for format %d", fmt);
+    uint32_t cold_from = split_point(g);
+    if (cold_from < g->n) {
+        GPR_TRY(cold_prepare(g, cold_from));
+        if (g->cold->ntiles == 0) cold_from = g->n;   // no cold edges
+    }
+    const bool split = cold_from < g->n;
     const uint32_t nblk = g->nbuckets + g->nlong_chunks;
-    GPR_TRY(ensure_partials(g, (size_t)nblk + g->nlong + 4096));
+    size_t need = (size_t)nblk + g->nlong + 4096;
+    if (split) need += g->cold->nbins + (size_t)g->cold->nmulti * (COLD_BIN / 1024);
+    GPR_TRY(ensure_partials(g, need));
     float *y = (float *)desc->prop->d;
     const void *cb = desc->contrib->d;
-    const double hb = tun().pr_hot_bytes / (double)fmt_bytes(fmt);
-    const uint32_t hot = hb >= (double)g->n ? g->n : (uint32_t)hb;
+    const float damping = desc->damping > 0.f ? desc->damping : 0.85f;
     GPR_HIP(hipEventRecord(g->ev0, g->stream));
-    {
+    if (split) {
+        ProfScope ps(g, "pr_cold_write");
+        GPR_TRY(cold_launch_write(g, fmt, cb));
+        hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, g->cold->mass,
+                           (const float *)nullptr, (const double *)nullptr, g->cold->nwchunks, g->dsum_slices,
+                           &g->ctr->ticket2, g->prs, 3, g->n, damping, 0);
+    }
+    if (cold_from > 0) {
+        // the pull over sources [0, cold_from); every source when not split
         ProfScope ps(g, "pr_pull");
 #define PR_PULL(F)                                                                                              \
     hipLaunchKernelGGL(k_pr_pull<F>, dim3(nblk), dim3(PR_THREADS), 0, g->stream, g->nbuckets, g->bucket,        \
                        g->cindex, g->cnbrs, g->deg, (const typename Codec<F>::store_t *)cb, y, g->long_rows,   \
-                       g->long_base, g->nlong, g->long_partial, g->partial, g->partial_f, g->prs, hot)
+                       g->long_base, g->nlong, g->long_partial, g->partial, g->partial_f, g->prs, cold_from)
         switch (fmt) {
         case GPR_CONTRIB_F32: PR_PULL(GPR_CONTRIB_F32); break;
         case GPR_CONTRIB_BF16: PR_PULL(GPR_CONTRIB_BF16); break;
@@ -435,15 +418,23 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
         }
 #undef PR_PULL
     }
+    if (cold_from > 0 && g->nlong) {
+        ProfScope ps(g, "pr_long");
+        hipLaunchKernelGGL(k_pr_long, dim3(grid_for(g->nlong, 256, 1024)), dim3(256), 0, g->stream, g->nlong,
+                           g->long_rows, g->long_base, g->long_partial, g->deg, y, g->partial, g->partial_f, nblk,
+                           g->prs, fmt == GPR_CONTRIB_F16);
+    }
+    uint32_t nparts = nblk + g->nlong;
+    if (split) {
+        // y (+)= the binned sums; their per-bin partials replace the pull's
+        ProfScope ps(g, "pr_cold_accum");
+        GPR_TRY(cold_launch_accum(g, fmt, y, cold_from > 0 ? 1 : 0, g->partial, g->partial_f, &nparts));
+    }
     {
         ProfScope ps(g, "pr_reduce");
-        if (g->nlong)
-            hipLaunchKernelGGL(k_pr_long, dim3(grid_for(g->nlong, 256, 1024)), dim3(256), 0, g->stream, g->nlong,
-                               g->long_rows, g->long_base, g->long_partial, g->deg, y, g->partial, g->partial_f,
-                               nblk, g->prs, fmt == GPR_CONTRIB_F16);
         hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, g->partial, g->partial_f,
-                           (const double *)nullptr, nblk + g->nlong, g->dsum_slices, &g->ctr->ticket2, g->prs, 0,
-                           g->n, desc->damping > 0.f ? desc->damping : 0.85f, fmt == GPR_CONTRIB_F16);
+                           (const double *)nullptr, nparts, g->dsum_slices, &g->ctr->ticket2, g->prs, 0, g->n,
+                           damping, fmt == GPR_CONTRIB_F16);
     }
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipEventRecord(g->ev1, g->stream));

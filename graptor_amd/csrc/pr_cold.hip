@@ -1,0 +1,557 @@
+// pr_cold.hip -- PageRank edges from "cold" sources, by propagation blocking.
+//
+// A pull gathers contrib[s] for every in-edge (s, d).  After degree-ordered
+// relabelling the contributions of the first sources stay in the L2, but a
+// gather from any other source is a random read that costs a whole cache
+// line (64 B for a 4 B value); those gathers bound the plain pull.  The
+// edges whose source is at or above a split point `from` (all edges when
+// from = 0) are therefore summed by streaming instead:
+//
+//   k_cold_write    per chunk of a tile of consecutive sources: the tile's
+//                   contributions are staged in LDS; its entries are read in
+//                   (bin, destination) order (a coalesced 4 B word: bin and
+//                   source offset) and each value is written to its slot in
+//                   the bin-major value array, where the tile's entries of
+//                   one bin form a contiguous run; the chunk's
+//                   |contribution| mass is summed.
+//   k_cold_accum    per chunk of a bin of COLD_BIN destinations: the bin's
+//                   entries (values and 2 B destinations) are contiguous and
+//                   streamed; equal destinations next to each other (runs
+//                   are sorted by destination) are combined across the wave,
+//                   and the sums go into LDS as 64-bit fixed-point integers,
+//                   so the result does not depend on the order of the
+//                   atomics.  A bin in one chunk updates y directly; the
+//                   chunks of a heavy bin (the hub destinations) write
+//                   integer partials that
+//   k_cold_combine  adds up, per destination.
+//
+// The fixed-point unit 2^E is chosen per iteration from the total mass M (a
+// deterministic FP64 sum of the write chunks' masses): every partial sum is
+// at most M <= 2^61 units, and a contribution loses less than one unit.
+// Write chunks go to the XCDs in contiguous ranges of tiles, so that the
+// runs of neighbouring tiles, which share cache lines at their ends, are
+// written through the same L2.
+//
+// The layout is static for a graph and split point: built once on the
+// device (count per (tile, bin), a radix sort of the entries by
+// (tile, bin, destination)), planned on the host (run offsets, chunks) and
+// kept until the graph's CSR changes.  Entries are the CSR edges s -> d with
+// s >= from, so directed graphs work the same way.
+#include "internal.hpp"
+#include "device.hpp"
+#include "pr_codec.hpp"
+
+#include <cstring>
+#include <rocprim/rocprim.hpp>
+
+#include <vector>
+
+namespace gpr {
+
+// ---- layout construction ---------------------------------------------------
+__device__ __forceinline__ uint32_t tile_of(const uint32_t *tile_lo, uint32_t ntiles, uint32_t u) {
+    uint32_t lo = 0, hi = ntiles;   // last t with tile_lo[t] <= u
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (tile_lo[mid] <= u) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// One thread per source row.  mode 0: count entries per (tile, bin).
+// mode 1: write the sort key (tile*nbins + bin) << 14 | (destination in
+// bin) and the source offset in the tile at the edge's position.
+__global__ void k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
+                            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt,
+                            uint64_t *keys, uint16_t *vals) {
+    const uint32_t first = tile_lo[0], last = tile_lo[ntiles];
+    const uint64_t ebase = index[first];
+    for (uint64_t u = (uint64_t)first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < last;
+         u += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e0 = index[u], e1 = index[u + 1];
+        if (e0 == e1) continue;
+        const uint32_t t = tile_of(tile_lo, ntiles, (uint32_t)u);
+        const uint16_t so = (uint16_t)(u - tile_lo[t]);
+        for (uint64_t e = e0; e < e1; ++e) {
+            const uint32_t v = nbrs[e];
+            const uint32_t b = v / COLD_BIN;
+            const uint64_t cell = (uint64_t)t * nbins + b;
+            if (mode == 0) {
+                atomicAdd(&cnt[cell], 1u);
+            } else {
+                keys[e - ebase] = (cell << 14) | (v - b * COLD_BIN);
+                vals[e - ebase] = so;
+            }
+        }
+    }
+}
+
+// Sorted entry p (tile-major): the write kernel's word bin << 15 | source
+// offset, and the destination in bin at the entry's bin-major slot.
+__global__ void k_cold_finish(const uint64_t *__restrict__ keys, const uint16_t *__restrict__ so, uint64_t count,
+                              uint32_t nbins, const uint32_t *__restrict__ delta, uint32_t *wsrc, uint16_t *cdst) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < count; p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[p];
+        const uint64_t cell = k >> 14;
+        const uint32_t b = (uint32_t)(cell % nbins);
+        wsrc[p] = (b << 15) | so[p];
+        cdst[(uint32_t)p + delta[cell]] = (uint16_t)(k & (COLD_BIN - 1));
+    }
+}
+
+void cold_free(gpr_graph *g) {
+    ColdLayout *L = g->cold;
+    if (!L) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    dev_free(L->tile_lo);
+    dev_free(L->delta);
+    dev_free(L->wsrc);
+    dev_free(L->cdst);
+    dev_free(L->vals);
+    dev_free(L->mass);
+    dev_free(L->wchunk);
+    dev_free(L->achunk);
+    dev_free(L->multi);
+    dev_free(L->partials);
+    delete L;
+    g->cold = nullptr;
+}
+
+// Tiles over [from, n): each starts at a source with edges and grows while
+// it spans < COLD_TILE_V sources and its edges stay <= COLD_TILE_E (a
+// single source with more edges gets a tile of its own).  lo[t+1] is the
+// start of the next tile (or the end); sources without edges between two
+// tiles belong to neither.
+static void make_tiles(const std::vector<uint64_t> &idx, uint32_t from, uint32_t n, std::vector<uint32_t> &lo) {
+    lo.clear();
+    uint32_t u = from;
+    while (u < n) {
+        while (u < n && idx[u + 1 - from] == idx[u - from]) ++u;   // skip sources without edges
+        if (u >= n) break;
+        const uint32_t t0 = u;
+        uint64_t e = 0;
+        while (u < n && u - t0 < COLD_TILE_V) {
+            const uint64_t d = idx[u + 1 - from] - idx[u - from];
+            if (e > 0 && e + d > COLD_TILE_E) break;
+            e += d;
+            ++u;
+        }
+        lo.push_back(t0);
+    }
+    lo.push_back(u);   // end of the last tile
+}
+
+static uint32_t bits_for(uint64_t v) {
+    uint32_t b = 1;
+    while (b < 64 && (1ull << b) <= v) ++b;
+    return b;
+}
+
+int cold_prepare(gpr_graph *g, uint32_t from) {
+    if (g->cold && g->cold->from == from) return GPR_OK;
+    cold_free(g);
+    ColdLayout *L = new ColdLayout();
+    L->from = from;
+    g->cold = L;
+    const uint32_t n = g->n;
+    if (from >= n) return GPR_OK;   // nothing cold: ntiles = 0
+    const uint32_t nbins = (uint32_t)(((uint64_t)n + COLD_BIN - 1) / COLD_BIN);
+    // the write kernel's word holds the bin in 17 bits
+    if (nbins > (1u << 17)) return GPR_OK;
+    std::vector<uint64_t> idx((size_t)(n - from) + 1);
+    GPR_HIP(hipMemcpyAsync(idx.data(), g->index + from, sizeof(uint64_t) * idx.size(), hipMemcpyDeviceToHost,
+                           g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    const uint64_t ncold = idx.back() - idx.front();
+    // no cold edges, or more than 32-bit entry positions address: the pull
+    // handles every edge
+    if (ncold == 0 || ncold >= 0xffffffffull) return GPR_OK;
+    std::vector<uint32_t> lo;
+    make_tiles(idx, from, n, lo);
+    idx.clear();
+    idx.shrink_to_fit();
+    const uint32_t ntiles = (uint32_t)lo.size() - 1;
+    const size_t cells = (size_t)ntiles * nbins;
+    uint32_t *cnt = nullptr;
+    uint64_t *k0 = nullptr, *k1 = nullptr;
+    uint16_t *v0 = nullptr, *v1 = nullptr;
+    void *tmp = nullptr;
+    int r = GPR_OK;
+    std::vector<uint32_t> hc, delta(cells);
+    std::vector<uint32_t> wch, ach, multi;   // plans, 4 words per item
+    uint32_t nslots = 0;
+    do {
+        if ((r = dev_alloc((void **)&L->tile_lo, sizeof(uint32_t) * lo.size(), "cold tiles"))) break;
+        if ((r = dev_alloc((void **)&cnt, sizeof(uint32_t) * cells, "cold counts"))) break;
+        if (hipMemcpyAsync(L->tile_lo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        const unsigned pg = grid_for(lo.back() - lo.front(), 256, 65536);
+        hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 0, g->index, g->nbrs, L->tile_lo, ntiles,
+                           nbins, cnt, (uint64_t *)nullptr, (uint16_t *)nullptr);
+        hc.resize(cells);
+        if (hipMemcpyAsync(hc.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(cnt);
+        cnt = nullptr;
+        // ---- host plan: tile-major run positions, bin-major slots, chunks
+        {
+            std::vector<uint64_t> bsize(nbins, 0), bstart(nbins + 1, 0);
+            for (uint32_t t = 0; t < ntiles; ++t)
+                for (uint32_t b = 0; b < nbins; ++b) bsize[b] += hc[(size_t)t * nbins + b];
+            for (uint32_t b = 0; b < nbins; ++b) bstart[b + 1] = bstart[b] + bsize[b];
+            std::vector<uint64_t> bpos(bstart.begin(), bstart.end() - 1);   // next slot of each bin
+            uint64_t pos = 0;
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                const uint64_t t0 = pos;
+                for (uint32_t b = 0; b < nbins; ++b) {
+                    const uint32_t c = hc[(size_t)t * nbins + b];
+                    delta[(size_t)t * nbins + b] = (uint32_t)(bpos[b] - pos);   // modulo 2^32
+                    bpos[b] += c;
+                    pos += c;
+                }
+                for (uint64_t p = t0; p < pos; p += COLD_TILE_E) {
+                    wch.push_back(t);
+                    wch.push_back((uint32_t)p);
+                    wch.push_back((uint32_t)std::min<uint64_t>(pos, p + COLD_TILE_E));
+                    wch.push_back(0);
+                }
+            }
+            for (uint32_t b = 0; b < nbins; ++b) {
+                const uint64_t e0 = bstart[b], e1 = bstart[b + 1];
+                const uint32_t k = e1 > e0 ? (uint32_t)((e1 - e0 + COLD_CHUNK_E - 1) / COLD_CHUNK_E) : 1u;
+                for (uint32_t i = 0; i < k; ++i) {
+                    ach.push_back(b);
+                    ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)i * COLD_CHUNK_E));
+                    ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)(i + 1) * COLD_CHUNK_E));
+                    ach.push_back(k == 1 ? COLD_DIRECT : nslots + i);
+                }
+                if (k > 1) {
+                    multi.push_back(b);
+                    multi.push_back(nslots);
+                    multi.push_back(k);
+                    multi.push_back(0);
+                    nslots += k;
+                }
+            }
+        }
+        hc.clear();
+        hc.shrink_to_fit();
+        if ((r = dev_alloc((void **)&L->delta, sizeof(uint32_t) * cells, "cold run offsets"))) break;
+        if (hipMemcpyAsync(L->delta, delta.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        // ---- entries sorted by (tile, bin, destination in bin)
+        if ((r = dev_alloc((void **)&k0, 8 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&k1, 8 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&v0, 2 * ncold, "cold sources"))) break;
+        if ((r = dev_alloc((void **)&v1, 2 * ncold, "cold sources"))) break;
+        hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 1, g->index, g->nbrs, L->tile_lo, ntiles,
+                           nbins, (uint32_t *)nullptr, k0, v0);
+        rocprim::double_buffer<uint64_t> kb(k0, k1);
+        rocprim::double_buffer<uint16_t> vb(v0, v1);
+        const unsigned end_bit = 14 + bits_for(cells);
+        size_t tsz = 0;
+        if (rocprim::radix_sort_pairs(nullptr, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
+        if ((r = dev_alloc(&tmp, tsz, "cold sort workspace"))) break;
+        if (rocprim::radix_sort_pairs(tmp, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
+        dev_free(tmp);
+        tmp = nullptr;
+        if ((r = dev_alloc((void **)&L->wsrc, 4 * ncold, "cold write words"))) break;
+        if ((r = dev_alloc((void **)&L->cdst, 2 * ncold, "cold destinations"))) break;
+        hipLaunchKernelGGL(k_cold_finish, dim3(grid_for(ncold, 256, 65536)), dim3(256), 0, g->stream, kb.current(),
+                           vb.current(), ncold, nbins, L->delta, L->wsrc, L->cdst);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(k0);
+        dev_free(k1);
+        dev_free(v0);
+        dev_free(v1);
+        k0 = k1 = nullptr;
+        v0 = v1 = nullptr;
+        if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * ncold, "cold values"))) break;
+        L->nwchunks = (uint32_t)(wch.size() / 4);
+        L->nachunks = (uint32_t)(ach.size() / 4);
+        L->nmulti = (uint32_t)(multi.size() / 4);
+        if ((r = dev_alloc((void **)&L->wchunk, sizeof(uint32_t) * wch.size(), "cold write chunks"))) break;
+        if ((r = dev_alloc((void **)&L->achunk, sizeof(uint32_t) * ach.size(), "cold chunks"))) break;
+        if ((r = dev_alloc((void **)&L->mass, sizeof(double) * L->nwchunks, "cold mass"))) break;
+        if (L->nmulti) {
+            if ((r = dev_alloc((void **)&L->multi, sizeof(uint32_t) * multi.size(), "cold multi-chunk bins"))) break;
+            if ((r = dev_alloc((void **)&L->partials, sizeof(long long) * COLD_BIN * nslots, "cold partials"))) break;
+            if (hipMemcpy(L->multi, multi.data(), sizeof(uint32_t) * multi.size(), hipMemcpyHostToDevice) !=
+                hipSuccess) { r = GPR_E_HIP; break; }
+        }
+        if (hipMemcpy(L->wchunk, wch.data(), sizeof(uint32_t) * wch.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(L->achunk, ach.data(), sizeof(uint32_t) * ach.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
+    } while (0);
+    dev_free(cnt);
+    dev_free(k0);
+    dev_free(k1);
+    dev_free(v0);
+    dev_free(v1);
+    dev_free(tmp);
+    if (r != GPR_OK) {
+        if (r == GPR_E_HIP) set_error("PageRank cold-edge layout build failed");
+        cold_free(g);
+        return r;
+    }
+    L->ntiles = ntiles;
+    L->nbins = nbins;
+    L->ncold = ncold;
+    return GPR_OK;
+}
+
+// ---- per-iteration kernels -------------------------------------------------
+constexpr int CW_THREADS = 1024;
+constexpr int CW_UNROLL = 8;
+
+// Workgroup i -> chunk: the XCD that runs workgroup i (i mod 8 on this
+// part) takes a contiguous range of chunks, in order.
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t count) {
+    const uint32_t q = count / 8, r = count % 8, x = i % 8, k = i / 8;
+    return x * q + min(x, r) + k;
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(CW_THREADS)
+k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
+             uint32_t nbins, const uint32_t *__restrict__ wsrc, const uint32_t *__restrict__ delta,
+             const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
+             double *__restrict__ mass, const PrScalars *__restrict__ prs) {
+    using C = Codec<FMT>;
+    using S = typename C::store_t;
+    __shared__ S win[COLD_TILE_V];
+    __shared__ double red[CW_THREADS / 64];
+    const uint32_t c = xcd_chunk(blockIdx.x, nwchunks);
+    const uint32_t t = wchunk[4 * c], p0 = wchunk[4 * c + 1], p1 = wchunk[4 * c + 2];
+    // a tile's sources span < COLD_TILE_V IDs; tile_lo[t+1] may lie further
+    // on when sources without edges follow the tile
+    const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
+    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) win[i] = contrib[lo + i];
+    __syncthreads();
+    const uint32_t *dt = delta + (size_t)t * nbins;
+    double ms = 0.0;
+    for (uint32_t base = p0; base < p1; base += CW_THREADS * CW_UNROLL) {
+        uint32_t wv[CW_UNROLL];
+#pragma unroll
+        for (int u = 0; u < CW_UNROLL; ++u) {
+            const uint32_t p = base + u * CW_THREADS + threadIdx.x;
+            wv[u] = p < p1 ? __builtin_nontemporal_load(&wsrc[p]) : 0u;
+        }
+        uint32_t dl[CW_UNROLL];
+#pragma unroll
+        for (int u = 0; u < CW_UNROLL; ++u) dl[u] = dt[wv[u] >> 15];
+#pragma unroll
+        for (int u = 0; u < CW_UNROLL; ++u) {
+            const uint32_t p = base + u * CW_THREADS + threadIdx.x;
+            if (p < p1) {
+                const S v = win[wv[u] & 0x7fffu];
+                vals[p + dl[u]] = v;
+                ms += (double)fabsf(C::dec(v));
+            }
+        }
+    }
+    ms = block_sum(ms, red);
+    if (threadIdx.x == 0) mass[c] = FMT == GPR_CONTRIB_F16 ? ms / (double)prs->enc_scale : ms;
+}
+
+constexpr int CA_THREADS = 1024;
+constexpr int CA_U = 8;                          // 64-entry blocks of one wave in flight
+
+// y[d] (+)= s * unit, and the running sum of y and maximum of y/deg.
+__device__ __forceinline__ void cold_store(float *y, uint32_t d, long long s, double unit, int add, double &my_sum,
+                                           float &my_max, const uint32_t *deg, bool f16) {
+    const double c = (double)s * unit;
+    const float yv = (float)(add ? (double)y[d] + c : c);
+    y[d] = yv;
+    my_sum += (double)yv;
+    if (f16) {
+        const uint32_t dg = deg[d];
+        if (dg) my_max = fmaxf(my_max, yv / (float)dg);
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void block_emit(double my_sum, float my_max, double *part, float *part_max, uint32_t slot,
+                                           double *dred, float *fred) {
+    const double ts = block_sum(my_sum, dred);
+    my_max = wave_max(my_max);
+    __syncthreads();
+    if (lane() == 0) fred[threadIdx.x / 64] = my_max;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mx = fred[0];
+        for (int i = 1; i < NT / 64; ++i) mx = fmaxf(mx, fred[i]);
+        part[slot] = ts;
+        part_max[slot] = mx;
+    }
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(CA_THREADS)
+k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
+             const uint16_t *__restrict__ cdst, const typename Codec<FMT>::store_t *__restrict__ vals,
+             const uint32_t *__restrict__ deg, float *__restrict__ y, long long *__restrict__ partials, int add,
+             double *__restrict__ part, float *__restrict__ part_max, const PrScalars *__restrict__ prs) {
+    using C = Codec<FMT>;
+    __shared__ unsigned long long acc[COLD_BIN];
+    __shared__ double dred[CA_THREADS / 64];
+    __shared__ float fred[CA_THREADS / 64];
+    const uint32_t b = achunk[4 * blockIdx.x], p0 = achunk[4 * blockIdx.x + 1], p1 = achunk[4 * blockIdx.x + 2];
+    const uint32_t slot = achunk[4 * blockIdx.x + 3];
+    for (uint32_t i = threadIdx.x; i < COLD_BIN; i += CA_THREADS) acc[i] = 0ull;
+    __syncthreads();
+    const double unit = prs->cold_unit;
+    const double inv = 1.0 / unit;   // a power of two: exact
+    const float dscale = FMT == GPR_CONTRIB_F16 ? 1.0f / prs->enc_scale : 1.0f;
+    const unsigned w = threadIdx.x / 64, ln = lane(), nw = CA_THREADS / 64;
+    for (uint32_t base = p0 + w * 64; base < p1; base += nw * 64 * CA_U) {
+        uint32_t key[CA_U];
+        typename C::store_t raw[CA_U];
+        long long fx[CA_U];
+#pragma unroll
+        for (int u = 0; u < CA_U; ++u) {
+            // unconditional loads (masked lanes read entry p0, in bounds as
+            // base < p1), so that all of them are in flight together
+            const uint32_t f = base + u * nw * 64 + ln;
+            const uint32_t fc = f < p1 ? f : p0;
+            key[u] = __builtin_nontemporal_load(&cdst[fc]);
+            raw[u] = __builtin_nontemporal_load(&vals[fc]);
+        }
+#pragma unroll
+        for (int u = 0; u < CA_U; ++u) {
+            const bool ok = base + u * nw * 64 + ln < p1;
+            key[u] = ok ? key[u] : 0xffffffffu;
+            const float v = ok ? C::dec(raw[u]) : 0.0f;
+            fx[u] = (long long)((double)(v * dscale) * inv);
+        }
+#pragma unroll
+        for (int u = 0; u < CA_U; ++u) {
+            const uint32_t nk = __shfl_down(key[u], 1, 64);
+            const bool dup = ln < 63 && nk == key[u] && key[u] != 0xffffffffu;
+            bool last = ln == 63 || nk != key[u];
+            if (__ballot(dup)) {
+                // a run is sorted by destination, so equal keys mostly sit
+                // next to each other: sum each stretch of equal adjacent keys
+                // across the wave (a segmented scan bounded by the stretch's
+                // first lane), one atomic at its end.  Equal keys of
+                // different stretches (a block spans runs) stay separate.
+                const uint32_t pk = __shfl_up(key[u], 1, 64);
+                const uint64_t heads = __ballot(ln == 0 || pk != key[u]);
+                const uint64_t upto = ln == 63 ? ~0ull : ((2ull << ln) - 1);
+                const unsigned h = 63 - __builtin_clzll(heads & upto);
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const long long tv = __shfl_up(fx[u], o, 64);
+                    if (ln >= h + (unsigned)o) fx[u] += tv;
+                }
+            } else {
+                last = true;
+            }
+            if (key[u] != 0xffffffffu && last && fx[u] != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx[u]);
+        }
+    }
+    __syncthreads();
+    const uint32_t d0 = b * COLD_BIN, nd = min(COLD_BIN, n - d0);
+    if (slot != COLD_DIRECT) {
+        // one of several chunks of this bin: integer partials for the combine
+        long long *pp = partials + (size_t)slot * COLD_BIN;
+        for (uint32_t i = threadIdx.x; i < COLD_BIN; i += CA_THREADS) pp[i] = (long long)acc[i];
+        if (threadIdx.x == 0) {
+            part[b] = 0.0;
+            part_max[b] = 0.0f;
+        }
+        return;
+    }
+    double my_sum = 0.0;
+    float my_max = 0.0f;
+    for (uint32_t i = threadIdx.x; i < nd; i += CA_THREADS)
+        cold_store(y, d0 + i, (long long)acc[i], unit, add, my_sum, my_max, deg, FMT == GPR_CONTRIB_F16);
+    block_emit<CA_THREADS>(my_sum, my_max, part, part_max, b, dred, fred);
+}
+
+constexpr int CC_THREADS = 1024;
+constexpr uint32_t CC_PER_BIN = COLD_BIN / CC_THREADS;
+
+// Bins in several chunks: one workgroup per CC_THREADS destinations sums
+// the chunks' integer partials; its y sum and y/deg maximum go to slot
+// nbins + blockIdx.x.
+__global__ void __launch_bounds__(CC_THREADS)
+k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, const long long *__restrict__ partials,
+               const uint32_t *__restrict__ deg, float *__restrict__ y, int add, double *__restrict__ part,
+               float *__restrict__ part_max, const PrScalars *__restrict__ prs, int f16) {
+    __shared__ double dred[CC_THREADS / 64];
+    __shared__ float fred[CC_THREADS / 64];
+    const uint32_t mb = blockIdx.x / CC_PER_BIN, sub = blockIdx.x % CC_PER_BIN;
+    const uint32_t b = multi[4 * mb], first = multi[4 * mb + 1], k = multi[4 * mb + 2];
+    const uint32_t i = sub * CC_THREADS + threadIdx.x;
+    const uint32_t d0 = b * COLD_BIN, nd = min(COLD_BIN, n - d0);
+    double my_sum = 0.0;
+    float my_max = 0.0f;
+    if (i < nd) {
+        long long s = 0;
+        uint32_t c = 0;
+        for (; c + 8 <= k; c += 8) {
+            long long t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = partials[(size_t)(first + c + j) * COLD_BIN + i];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) s += t[j];
+        }
+        for (; c < k; ++c) s += partials[(size_t)(first + c) * COLD_BIN + i];
+        cold_store(y, d0 + i, s, prs->cold_unit, add, my_sum, my_max, deg, f16 != 0);
+    }
+    block_emit<CC_THREADS>(my_sum, my_max, part, part_max, nbins + blockIdx.x, dred, fred);
+}
+
+int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
+    ColdLayout *L = g->cold;
+#define CW(F)                                                                                                   \
+    hipLaunchKernelGGL(k_cold_write<F>, dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,           \
+                       L->nwchunks, L->tile_lo, L->nbins, L->wsrc, L->delta,                                    \
+                       (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, L->mass, \
+                       g->prs)
+    switch (fmt) {
+    case GPR_CONTRIB_F32: CW(GPR_CONTRIB_F32); break;
+    case GPR_CONTRIB_BF16: CW(GPR_CONTRIB_BF16); break;
+    case GPR_CONTRIB_F16: CW(GPR_CONTRIB_F16); break;
+    case GPR_CONTRIB_CFP16: CW(GPR_CONTRIB_CFP16); break;
+    case GPR_CONTRIB_BF16T: CW(GPR_CONTRIB_BF16T); break;
+    }
+#undef CW
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
+
+int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, float *part_max, uint32_t *nparts) {
+    ColdLayout *L = g->cold;
+#define CA(F)                                                                                                    \
+    hipLaunchKernelGGL(k_cold_accum<F>, dim3(L->nachunks), dim3(CA_THREADS), 0, g->stream, L->achunk, g->n,       \
+                       L->cdst, (const typename Codec<F>::store_t *)L->vals, g->deg, y, L->partials, add, part,   \
+                       part_max, g->prs)
+    switch (fmt) {
+    case GPR_CONTRIB_F32: CA(GPR_CONTRIB_F32); break;
+    case GPR_CONTRIB_BF16: CA(GPR_CONTRIB_BF16); break;
+    case GPR_CONTRIB_F16: CA(GPR_CONTRIB_F16); break;
+    case GPR_CONTRIB_CFP16: CA(GPR_CONTRIB_CFP16); break;
+    case GPR_CONTRIB_BF16T: CA(GPR_CONTRIB_BF16T); break;
+    }
+#undef CA
+    if (L->nmulti)
+        hipLaunchKernelGGL(k_cold_combine, dim3(L->nmulti * CC_PER_BIN), dim3(CC_THREADS), 0, g->stream, L->multi,
+                           L->nbins, g->n, (const long long *)L->partials, g->deg, y, add, part, part_max, g->prs,
+                           fmt == GPR_CONTRIB_F16 ? 1 : 0);
+    GPR_HIP(hipGetLastError());
+    *nparts = L->nbins + L->nmulti * CC_PER_BIN;
+    return GPR_OK;
+}
+
+} // namespace gpr

@@ -197,10 +197,10 @@ extern "C" int gpr_init(int device) {
 
 static double *tunable(const char *name) {
     Tunables &t = tun();
-    if (!strcmp(name, "pr_hot_bytes")) return &t.pr_hot_bytes;
     if (!strcmp(name, "pull_serial_max")) return &t.pull_serial_max;
     if (!strcmp(name, "pull_blocks_per_cu")) return &t.pull_blocks_per_cu;
     if (!strcmp(name, "push_blocks_per_cu")) return &t.push_blocks_per_cu;
+    if (!strcmp(name, "pr_split_bytes")) return &t.pr_split_bytes;
     return nullptr;
 }
 

@@ -318,8 +318,11 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
 
 /* Process-wide runtime tunables (no reference counterpart; chosen by the
  * autotuner, graptor_amd/autotune.py):
- *   pr_hot_bytes        contribution bytes gathered through L2 (the rest of
- *                       the gathers use non-temporal loads)
+ *   pr_split_bytes      PageRank: FP32 bytes of source contributions the
+ *                       pull gathers (the sources [0, bytes/4)); edges from
+ *                       the other sources are summed by streaming through a
+ *                       binned layout.  0 bins every edge; >= 4n pulls every
+ *                       edge.  Default 0
  *   pull_serial_max     pull rows up to this in-degree run lane-serially
  *   pull_blocks_per_cu  pull grid size per compute unit
  *   push_blocks_per_cu  push grid cap per compute unit */

@@ -1,0 +1,162 @@
+"""PageRank with the cold-source split (pr_cold.hip) against the FP64 oracle.
+
+The engine pulls the contributions of sources below a split point and sums
+the edges from the other ("cold") sources through a binned layout with
+fixed-point accumulation (split point 0: every edge binned).  The split is
+set here per test (tunable pr_split_bytes = 4 bytes per pulled source;
+1e12 = no split) so that test-sized graphs take the binned path, with tiles
+capped by source count (32768) and by edges (262144), several destination
+bins (16384 each), long rows (> 2048 in-edges), directed graphs, and (on
+RMAT-20) hub bins of more than 2^20 entries that are summed in chunks.
+
+Bars: the same as the plain pull (relative L1 vs FP64: FP32 1e-6, 16-bit
+formats 1e-3 on RMAT and 2e-3 on near-regular graphs, truncating bf16
+1e-2); FP32 results within 5e-8 relative L1 of the unsplit engine (both
+sum in at least FP64 precision; they differ by final FP32 roundings); runs
+bit-identical to each other (fixed-point sums do not depend on the order of
+the atomics); sum of ranks 1 within 1e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+G = pytest.importorskip("graptor_amd")
+
+TOL = {"fp32": 1e-6, "bf16": 1e-3, "fp16": 1e-3, "cfp16": 1e-3, "bf16t": 1e-2}
+NEAR_REGULAR_FACTOR = 2.0
+
+
+def graphs():
+    out = {}
+    out["rmat14"] = O.rmat_graph(14)
+    out["rmat16_ef8_a045"] = O.rmat_graph(16, 8, (0.45, 0.25, 0.25))
+    out["grid"] = O.grid_graph(200, 180, 0.01, seed=5)
+    out["er"] = O.er_graph(40000, 6, seed=12)
+    # hub with 5000 leaves: the hub's row is cut into chunks (> 2048 edges)
+    # and every leaf is a cold source
+    out["star5000"] = O.build_csr(5001, np.zeros(5000, np.uint32), np.arange(1, 5001, dtype=np.uint32))
+    return out
+
+
+GRAPHS = graphs()
+
+
+@pytest.fixture(autouse=True)
+def restore_split():
+    G.lib()   # fails loudly when the library or the gfx950 device is missing
+    old = G.tune_get("pr_split_bytes")
+    yield
+    G.tune_set("pr_split_bytes", old)
+
+
+def split_at(h):
+    """Pull sources [0, h); sum the rest through the binned layout."""
+    G.tune_set("pr_split_bytes", 4.0 * h)
+
+
+def rel_l1(a, b):
+    return float(np.abs(a.astype(np.float64) - b).sum() / np.abs(b).sum())
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("reorder", [False, True], ids=["ids", "degree_order"])
+def test_split_fp32_vs_fp64_and_unsplit(name, reorder):
+    index, nbrs = GRAPHS[name]
+    n = len(index) - 1
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        if reorder:
+            g.reorder()
+        G.tune_set("pr_split_bytes", 1e12)
+        x0, _ = g.prv(max_iter=20)
+        for h in sorted({0, 1, 7, max(1, n // 3), n - 1}):
+            split_at(h)
+            x1, info = g.prv(max_iter=20)
+            x2, _ = g.prv(max_iter=20)
+            assert np.array_equal(x1, x2), (name, h)
+            assert rel_l1(x1, x64) <= 1e-6, (name, h, rel_l1(x1, x64))
+            assert rel_l1(x1, x0) <= 5e-8, (name, h, rel_l1(x1, x0))
+            assert abs(float(x1.sum(dtype=np.float64)) - 1.0) < 1e-6
+            assert info.iters == 20
+
+
+@pytest.mark.parametrize("fmt", sorted(TOL))
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+def test_split_formats_vs_fp64(name, fmt):
+    index, nbrs = GRAPHS[name]
+    n = len(index) - 1
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    tol = TOL[fmt] * (1.0 if name.startswith("rmat") or fmt == "fp32" else NEAR_REGULAR_FACTOR)
+    with G.Graph.from_csr(index, nbrs).reorder() as g:
+        for h in (0, max(1, n // 50)):
+            split_at(h)
+            x, info = g.pr_vary([(fmt, 0.0)], max_iter=20)
+            assert rel_l1(x, x64) <= tol, (name, fmt, h, rel_l1(x, x64))
+            xb, _ = g.pr_vary([(fmt, 0.0)], max_iter=20)
+            assert np.array_equal(x, xb), (name, fmt, h)
+            assert abs(float(x.sum(dtype=np.float64)) - 1.0) < 1e-6
+
+
+def test_split_converged_stages():
+    index, nbrs = GRAPHS["rmat16_ef8_a045"]
+    x64, it64, _ = O.pr_fp64(index, nbrs, 1000, 1e-7)
+    with G.Graph.from_csr(index, nbrs).reorder() as g:
+        split_at(1000)
+        x, info = g.prv(max_iter=1000, tol=1e-7)
+        assert info.delta <= 1e-7 and abs(info.iters - it64) <= 2
+        assert np.abs(x - x64).sum() < 1e-6
+        xs, infos = g.pr_vary([("cfp16", 1e-3), ("fp32", 1e-7)], max_iter=1000)
+        assert infos.delta <= 1e-7
+        assert np.abs(xs - x64).sum() < 2e-6
+
+
+def test_split_directed_graph():
+    rng = np.random.default_rng(8)
+    n = 50000
+    src = rng.integers(0, n, 400000).astype(np.uint32)
+    dst = (rng.pareto(1.2, 400000) * 50).astype(np.int64) % n
+    index, nbrs = O.build_csr(n, src, dst.astype(np.uint32), symmetrize=False)
+    ci, cn = O.build_csr(n, dst.astype(np.uint32), src, symmetrize=False)
+    x64, _, _ = O.pr_fp64(ci, cn, 20, 0.0, outdeg=np.diff(index).astype(np.uint32))
+    with G.Graph.from_csr(index, nbrs, symmetric=False) as g:
+        for h in (0, 1, 4096, 30000):
+            split_at(h)
+            x, _ = g.prv(max_iter=20)
+            assert rel_l1(x, x64) <= 1e-6, (h, rel_l1(x, x64))
+
+
+def test_split_point_changes_and_graph_reuse():
+    """The layout follows the split point; relabelling drops it."""
+    index, nbrs = GRAPHS["rmat14"]
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        for h in (100, 3, 0, 100, 5000):
+            split_at(h)
+            x, _ = g.prv(max_iter=20)
+            assert rel_l1(x, x64) <= 1e-6, h
+        g.reorder()
+        x, _ = g.prv(max_iter=20)
+        assert rel_l1(x, x64) <= 1e-6
+
+
+@pytest.mark.parametrize("h", [0, 1 << 16])
+def test_split_rmat20_chunked_hub_bins(h):
+    """RMAT-20 relabelled: the hub bins hold several million entries and are
+    summed in chunks of 2^20 whose integer partials are combined."""
+    with G.Graph.rmat(20) as g:
+        index, nbrs = g.download()
+        x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+        g.reorder()
+        G.tune_set("pr_split_bytes", 1e12)
+        x0, _ = g.prv(max_iter=20)
+        split_at(h)
+        x1, _ = g.prv(max_iter=20)
+        x2, _ = g.prv(max_iter=20)
+        assert np.array_equal(x1, x2)
+        assert rel_l1(x1, x64) <= 1e-6, rel_l1(x1, x64)
+        assert rel_l1(x1, x0) <= 5e-8, rel_l1(x1, x0)
+        xb, _ = g.pr_vary([("bf16", 0.0)], max_iter=20)
+        assert rel_l1(xb, x64) <= 1e-3, rel_l1(xb, x64)
