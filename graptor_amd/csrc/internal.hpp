@@ -92,8 +92,8 @@ struct PrScalars {
     float max_ratio;   // max over v of y[v]/deg[v] (for the FP16 scale)
     float enc_scale;   // contribution encode scale 2^k (FP16 only)
     float dec_scale;   // decode scale 2^-k of the stored contributions
-    float pad;
-    double cold_unit;  // fixed-point unit of the binned (cold-source) sums
+    int cold_exp;      // E of the fixed-point unit 2^E of the binned sums
+    double cold_unit;  // that unit, 2^E
 };
 
 // Static propagation-blocking layout of the PageRank edges whose source is

@@ -285,9 +285,10 @@ k_pr_reduce(const double *__restrict__ part, const float *__restrict__ part_max,
     } else if (mode == 3) {
         // fixed-point unit of the cold sums: 2^E with M <= 2^61 units, M the
         // total |contribution| over the cold edges
-        double unit = 1.0;
-        if (a > 0.0 && a < 1e300) unit = ldexp(1.0, ilogb(a) + 1 - 61);
-        prs->cold_unit = unit;
+        int e = 0;
+        if (a > 0.0 && a < 1e300) e = ilogb(a) + 1 - 61;
+        prs->cold_exp = e;
+        prs->cold_unit = ldexp(1.0, e);
     }
     // mode 2 (maximum only): the slices are read by k_pr_set_scale
     *ticket = 0;

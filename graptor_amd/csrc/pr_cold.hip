@@ -312,6 +312,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
 // ---- per-iteration kernels -------------------------------------------------
 constexpr int CW_THREADS = 1024;
 constexpr int CW_UNROLL = 8;
+constexpr uint32_t CW_DELTA_LDS = 4096;   // run offsets staged in LDS up to this many bins
 
 // Workgroup i -> chunk: the XCD that runs workgroup i (i mod 8 on this
 // part) takes a contiguous range of chunks, in order.
@@ -320,31 +321,21 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t count) {
     return x * q + min(x, r) + k;
 }
 
-template <int FMT>
-__global__ void __launch_bounds__(CW_THREADS)
-k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
-             uint32_t nbins, const uint32_t *__restrict__ wsrc, const uint32_t *__restrict__ delta,
-             const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
-             double *__restrict__ mass, const PrScalars *__restrict__ prs) {
-    using C = Codec<FMT>;
-    using S = typename C::store_t;
-    __shared__ S win[COLD_TILE_V];
-    __shared__ double red[CW_THREADS / 64];
-    const uint32_t c = xcd_chunk(blockIdx.x, nwchunks);
-    const uint32_t t = wchunk[4 * c], p0 = wchunk[4 * c + 1], p1 = wchunk[4 * c + 2];
-    // a tile's sources span < COLD_TILE_V IDs; tile_lo[t+1] may lie further
-    // on when sources without edges follow the tile
-    const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
-    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) win[i] = contrib[lo + i];
-    __syncthreads();
-    const uint32_t *dt = delta + (size_t)t * nbins;
+// The copy loop of one write chunk: entries [p0, p1) of the tile, values
+// from the LDS window, run offsets from dt (LDS or global, fixed per
+// instantiation so that every access has a known address space).
+template <int FMT, class D>
+__device__ __forceinline__ double cold_copy(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ wsrc, D dt,
+                                            const typename Codec<FMT>::store_t *win,
+                                            typename Codec<FMT>::store_t *__restrict__ vals) {
+    using S = typename Codec<FMT>::store_t;
     double ms = 0.0;
     for (uint32_t base = p0; base < p1; base += CW_THREADS * CW_UNROLL) {
         uint32_t wv[CW_UNROLL];
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
-            wv[u] = p < p1 ? __builtin_nontemporal_load(&wsrc[p]) : 0u;
+            wv[u] = __builtin_nontemporal_load(&wsrc[p < p1 ? p : p0]);
         }
         uint32_t dl[CW_UNROLL];
 #pragma unroll
@@ -355,16 +346,57 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
             if (p < p1) {
                 const S v = win[wv[u] & 0x7fffu];
                 vals[p + dl[u]] = v;
-                ms += (double)fabsf(C::dec(v));
+                ms += (double)fabsf(Codec<FMT>::dec(v));
             }
         }
     }
+    return ms;
+}
+
+template <int FMT>
+__global__ void __launch_bounds__(CW_THREADS)
+k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
+             uint32_t nbins, const uint32_t *__restrict__ wsrc, const uint32_t *__restrict__ delta,
+             const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
+             double *__restrict__ mass, const PrScalars *__restrict__ prs) {
+    using S = typename Codec<FMT>::store_t;
+    __shared__ S win[COLD_TILE_V];
+    __shared__ uint32_t dls[CW_DELTA_LDS];
+    __shared__ double red[CW_THREADS / 64];
+    const uint32_t c = xcd_chunk(blockIdx.x, nwchunks);
+    const uint32_t t = wchunk[4 * c], p0 = wchunk[4 * c + 1], p1 = wchunk[4 * c + 2];
+    // a tile's sources span < COLD_TILE_V IDs; tile_lo[t+1] may lie further
+    // on when sources without edges follow the tile
+    const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
+    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) win[i] = contrib[lo + i];
+    const uint32_t *dt = delta + (size_t)t * nbins;
+    const bool staged = nbins <= CW_DELTA_LDS;   // the tile's run offsets fit in LDS
+    if (staged)
+        for (uint32_t i = threadIdx.x; i < nbins; i += CW_THREADS) dls[i] = dt[i];
+    __syncthreads();
+    double ms = staged ? cold_copy<FMT>(p0, p1, wsrc, (const uint32_t *)dls, win, vals)
+                       : cold_copy<FMT>(p0, p1, wsrc, dt, win, vals);
     ms = block_sum(ms, red);
     if (threadIdx.x == 0) mass[c] = FMT == GPR_CONTRIB_F16 ? ms / (double)prs->enc_scale : ms;
 }
 
 constexpr int CA_THREADS = 1024;
 constexpr int CA_U = 8;                          // 64-entry blocks of one wave in flight
+
+// v * 2^-E truncated toward zero, as an integer, from the float's bits:
+// v = m * 2^(e-150) for a normal float (m the 24-bit significand), so the
+// result is m shifted by e - 150 - E = e + shift.  The unit is chosen so
+// that |v| * 2^-E < 2^61 for every contribution, so the shift stays below
+// 38; zeros, subnormals (far below the unit) and non-finite values give 0.
+__device__ __forceinline__ long long to_fixed(float v, int shift) {
+    const uint32_t bits = __float_as_uint(v);
+    const int e = (int)((bits >> 23) & 0xffu);
+    if (e == 0 || e == 0xff) return 0;
+    const uint64_t m = (uint64_t)((bits & 0x7fffffu) | 0x800000u);
+    const int s = e + shift;
+    const uint64_t mag = s >= 0 ? (s < 40 ? m << s : 0ull) : (s > -25 ? m >> (-s) : 0ull);
+    return (bits >> 31) ? -(long long)mag : (long long)mag;
+}
 
 // y[d] (+)= s * unit, and the running sum of y and maximum of y/deg.
 __device__ __forceinline__ void cold_store(float *y, uint32_t d, long long s, double unit, int add, double &my_sum,
@@ -410,53 +442,31 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
     for (uint32_t i = threadIdx.x; i < COLD_BIN; i += CA_THREADS) acc[i] = 0ull;
     __syncthreads();
     const double unit = prs->cold_unit;
-    const double inv = 1.0 / unit;   // a power of two: exact
     const float dscale = FMT == GPR_CONTRIB_F16 ? 1.0f / prs->enc_scale : 1.0f;
     const unsigned w = threadIdx.x / 64, ln = lane(), nw = CA_THREADS / 64;
-    for (uint32_t base = p0 + w * 64; base < p1; base += nw * 64 * CA_U) {
-        uint32_t key[CA_U];
+    const int shift = -150 - prs->cold_exp;
+    // chunk-relative 32-bit offsets
+    const uint16_t *kb = cdst + p0;
+    const typename C::store_t *vb = vals + p0;
+    const uint32_t len = p1 - p0;
+    for (uint32_t base = w * 64; base < len; base += nw * 64 * CA_U) {
+        uint16_t key[CA_U];
         typename C::store_t raw[CA_U];
-        long long fx[CA_U];
+        // unconditional loads (masked lanes read entry 0, in bounds as
+        // base < len), so that all of them are in flight together
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            // unconditional loads (masked lanes read entry p0, in bounds as
-            // base < p1), so that all of them are in flight together
-            const uint32_t f = base + u * nw * 64 + ln;
-            const uint32_t fc = f < p1 ? f : p0;
-            key[u] = __builtin_nontemporal_load(&cdst[fc]);
-            raw[u] = __builtin_nontemporal_load(&vals[fc]);
+            const uint32_t o = base + u * nw * 64 + ln;
+            const uint32_t oc = o < len ? o : 0u;
+            key[u] = __builtin_nontemporal_load(&kb[oc]);
+            raw[u] = __builtin_nontemporal_load(&vb[oc]);
         }
+        // LDS atomics are exact whatever the order and whatever the number
+        // of lanes on one address (the hub destinations)
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            const bool ok = base + u * nw * 64 + ln < p1;
-            key[u] = ok ? key[u] : 0xffffffffu;
-            const float v = ok ? C::dec(raw[u]) : 0.0f;
-            fx[u] = (long long)((double)(v * dscale) * inv);
-        }
-#pragma unroll
-        for (int u = 0; u < CA_U; ++u) {
-            const uint32_t nk = __shfl_down(key[u], 1, 64);
-            const bool dup = ln < 63 && nk == key[u] && key[u] != 0xffffffffu;
-            bool last = ln == 63 || nk != key[u];
-            if (__ballot(dup)) {
-                // a run is sorted by destination, so equal keys mostly sit
-                // next to each other: sum each stretch of equal adjacent keys
-                // across the wave (a segmented scan bounded by the stretch's
-                // first lane), one atomic at its end.  Equal keys of
-                // different stretches (a block spans runs) stay separate.
-                const uint32_t pk = __shfl_up(key[u], 1, 64);
-                const uint64_t heads = __ballot(ln == 0 || pk != key[u]);
-                const uint64_t upto = ln == 63 ? ~0ull : ((2ull << ln) - 1);
-                const unsigned h = 63 - __builtin_clzll(heads & upto);
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const long long tv = __shfl_up(fx[u], o, 64);
-                    if (ln >= h + (unsigned)o) fx[u] += tv;
-                }
-            } else {
-                last = true;
-            }
-            if (key[u] != 0xffffffffu && last && fx[u] != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx[u]);
+            const long long fx = to_fixed(C::dec(raw[u]) * dscale, shift);
+            if (base + u * nw * 64 + ln < len && fx != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx);
         }
     }
     __syncthreads();
