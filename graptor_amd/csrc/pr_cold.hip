@@ -330,12 +330,13 @@ __device__ __forceinline__ double cold_copy(uint32_t p0, uint32_t p1, const uint
                                             typename Codec<FMT>::store_t *__restrict__ vals) {
     using S = typename Codec<FMT>::store_t;
     double ms = 0.0;
-    for (uint32_t base = p0; base < p1; base += CW_THREADS * CW_UNROLL) {
+    // from p0 rounded down to 64: every wave reads whole cache lines
+    for (uint32_t base = p0 & ~63u; base < p1; base += CW_THREADS * CW_UNROLL) {
         uint32_t wv[CW_UNROLL];
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
-            wv[u] = __builtin_nontemporal_load(&wsrc[p < p1 ? p : p0]);
+            wv[u] = __builtin_nontemporal_load(&wsrc[p >= p0 && p < p1 ? p : p0]);
         }
         uint32_t dl[CW_UNROLL];
 #pragma unroll
@@ -343,7 +344,7 @@ __device__ __forceinline__ double cold_copy(uint32_t p0, uint32_t p1, const uint
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
-            if (p < p1) {
+            if (p >= p0 && p < p1) {
                 const S v = win[wv[u] & 0x7fffu];
                 vals[p + dl[u]] = v;
                 ms += (double)fabsf(Codec<FMT>::dec(v));
@@ -445,19 +446,20 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
     const float dscale = FMT == GPR_CONTRIB_F16 ? 1.0f / prs->enc_scale : 1.0f;
     const unsigned w = threadIdx.x / 64, ln = lane(), nw = CA_THREADS / 64;
     const int shift = -150 - prs->cold_exp;
-    // chunk-relative 32-bit offsets
-    const uint16_t *kb = cdst + p0;
-    const typename C::store_t *vb = vals + p0;
-    const uint32_t len = p1 - p0;
+    // 32-bit offsets from the chunk's first entry rounded down to 64, so
+    // that every wave reads whole cache lines no other wave reads
+    const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
+    const uint16_t *kb = cdst + a0;
+    const typename C::store_t *vb = vals + a0;
     for (uint32_t base = w * 64; base < len; base += nw * 64 * CA_U) {
         uint16_t key[CA_U];
         typename C::store_t raw[CA_U];
-        // unconditional loads (masked lanes read entry 0, in bounds as
-        // base < len), so that all of them are in flight together
+        // unconditional loads (masked lanes read the chunk's first entry),
+        // so that all of them are in flight together
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
             const uint32_t o = base + u * nw * 64 + ln;
-            const uint32_t oc = o < len ? o : 0u;
+            const uint32_t oc = o >= lo && o < len ? o : lo;
             key[u] = __builtin_nontemporal_load(&kb[oc]);
             raw[u] = __builtin_nontemporal_load(&vb[oc]);
         }
@@ -465,8 +467,9 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
         // of lanes on one address (the hub destinations)
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
+            const uint32_t o = base + u * nw * 64 + ln;
             const long long fx = to_fixed(C::dec(raw[u]) * dscale, shift);
-            if (base + u * nw * 64 + ln < len && fx != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx);
+            if (o >= lo && o < len && fx != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx);
         }
     }
     __syncthreads();
