@@ -45,6 +45,16 @@ def pr_pull_bytes(n, m, s):
 PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 
 
+# HBM bytes of one PR edgemap (FP32, every edge binned) on relabelled RMAT-26
+# ef16, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the edgemap's
+# kernels (k_cold_write, k_cold_accum, k_cold_combine).  Each counter is
+# scaled by what it reads for k_pr_update<fp32>, a plain stream of 12n bytes
+# in and 8n out: FETCH_SIZE x 2.00, WRITE_SIZE x 1.00.  The summary with the
+# commands is profiles/pr_edgemap_rmat26_pmc.md.  Other configurations
+# report null.
+MEASURED_TRAFFIC = {(26, 16, 0.0): 30.753e9}
+
+
 def pr_edgemap_ms(g):
     """Per-scope average ms of the profiled PR edgemaps and their sum per
     edgemap (each scope runs at most once per edgemap)."""
@@ -143,7 +153,8 @@ def run(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC.get((args.scale, args.edge_factor, split))
+                        if not args.no_reorder else None),
             "algorithmic_bytes_per_launch": bytes_pull,
             "avg_launch_ms": avg_pull_ms,
             "kernels_avg_ms": per_scope,
