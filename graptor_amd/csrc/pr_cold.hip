@@ -58,31 +58,70 @@ __device__ __forceinline__ uint32_t tile_of(const uint32_t *tile_lo, uint32_t nt
     return lo;
 }
 
-// One thread per source row.  mode 0: count entries per (tile, bin).
-// mode 1: write the sort key (tile*nbins + bin) << 14 | (destination in
-// bin) and the source offset in the tile at the edge's position.
-__global__ void k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
-                            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt,
-                            uint64_t *keys, uint16_t *vals) {
+// Row of edge e: the last r in [lo, hi) with index[r] <= e.
+__device__ __forceinline__ uint32_t cold_row_of(const uint64_t *index, uint32_t lo, uint32_t hi, uint64_t e) {
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (index[mid] <= e) lo = mid + 1; else hi = mid;
+    }
+    return lo - 1;
+}
+
+// Over the cold edges in tiles of CK_EDGES consecutive edges (balanced
+// whatever the degrees; the rows of a tile found through LDS-staged offsets
+// when they fit).  mode 0: count entries per (tile, bin).  mode 1: write
+// the sort key (tile*nbins + bin) << 14 | (destination in bin) and the
+// source offset in the tile at the edge's position.
+constexpr int CK_EDGES = 4096;
+__global__ void __launch_bounds__(256)
+k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
+            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt, uint64_t *keys,
+            uint16_t *vals) {
+    __shared__ uint64_t sidx[CK_EDGES + 1];
+    __shared__ uint32_t s_r0, s_cnt, s_t0;
     const uint32_t first = tile_lo[0], last = tile_lo[ntiles];
-    const uint64_t ebase = index[first];
-    for (uint64_t u = (uint64_t)first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; u < last;
-         u += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t e0 = index[u], e1 = index[u + 1];
-        if (e0 == e1) continue;
-        const uint32_t t = tile_of(tile_lo, ntiles, (uint32_t)u);
-        const uint16_t so = (uint16_t)(u - tile_lo[t]);
-        for (uint64_t e = e0; e < e1; ++e) {
+    const uint64_t E0 = index[first], E1 = index[last];
+    const uint64_t nblk = (E1 - E0 + CK_EDGES - 1) / CK_EDGES;
+    for (uint64_t k = blockIdx.x; k < nblk; k += gridDim.x) {
+        const uint64_t e0 = E0 + k * CK_EDGES, e1 = min(E1, e0 + CK_EDGES);
+        if (threadIdx.x == 0) {
+            const uint32_t r0 = cold_row_of(index, first, last + 1, e0);
+            const uint32_t r1 = cold_row_of(index, r0, last + 1, e1 - 1);
+            s_r0 = r0;
+            s_cnt = r1 - r0 + 2;
+            s_t0 = tile_of(tile_lo, ntiles, r0);
+        }
+        __syncthreads();
+        const uint32_t r0 = s_r0, rc = s_cnt, t0 = s_t0;
+        const bool in_lds = rc <= CK_EDGES + 1;
+        if (in_lds)
+            for (uint32_t j = threadIdx.x; j < rc; j += 256) sidx[j] = index[r0 + j];
+        __syncthreads();
+        for (uint64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+            uint32_t u;
+            if (in_lds) {
+                uint32_t lo = 0, hi = rc;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (sidx[mid] <= e) lo = mid + 1; else hi = mid;
+                }
+                u = r0 + lo - 1;
+            } else {
+                u = cold_row_of(index, r0, r0 + rc, e);
+            }
+            uint32_t t = t0;   // the rows of one edge block span few tiles
+            while (t + 1 < ntiles && tile_lo[t + 1] <= u) ++t;
             const uint32_t v = nbrs[e];
             const uint32_t b = v / COLD_BIN;
             const uint64_t cell = (uint64_t)t * nbins + b;
             if (mode == 0) {
                 atomicAdd(&cnt[cell], 1u);
             } else {
-                keys[e - ebase] = (cell << 14) | (v - b * COLD_BIN);
-                vals[e - ebase] = so;
+                keys[e - E0] = (cell << 14) | (v - b * COLD_BIN);
+                vals[e - E0] = (uint16_t)(u - tile_lo[t]);
             }
         }
+        __syncthreads();
     }
 }
 
@@ -186,7 +225,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         if (hipMemcpyAsync(L->tile_lo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, g->stream) !=
             hipSuccess) { r = GPR_E_HIP; break; }
         if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        const unsigned pg = grid_for(lo.back() - lo.front(), 256, 65536);
+        const unsigned pg = grid_for(ncold, CK_EDGES, 65536);
         hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 0, g->index, g->nbrs, L->tile_lo, ntiles,
                            nbins, cnt, (uint64_t *)nullptr, (uint16_t *)nullptr);
         hc.resize(cells);
@@ -263,7 +302,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         dev_free(tmp);
         tmp = nullptr;
         if ((r = dev_alloc((void **)&L->wsrc, 4 * ncold, "cold write words"))) break;
-        if ((r = dev_alloc((void **)&L->cdst, 2 * ncold, "cold destinations"))) break;
+        if ((r = dev_alloc((void **)&L->cdst, 2 * (ncold + 64), "cold destinations"))) break;
         hipLaunchKernelGGL(k_cold_finish, dim3(grid_for(ncold, 256, 65536)), dim3(256), 0, g->stream, kb.current(),
                            vb.current(), ncold, nbins, L->delta, L->wsrc, L->cdst);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
@@ -273,7 +312,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         dev_free(v1);
         k0 = k1 = nullptr;
         v0 = v1 = nullptr;
-        if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * ncold, "cold values"))) break;
+        if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * (ncold + 64), "cold values"))) break;
         L->nwchunks = (uint32_t)(wch.size() / 4);
         L->nachunks = (uint32_t)(ach.size() / 4);
         L->nmulti = (uint32_t)(multi.size() / 4);
@@ -451,6 +490,34 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
     const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
     const uint16_t *kb = cdst + a0;
     const typename C::store_t *vb = vals + a0;
+    if constexpr (sizeof(typename C::store_t) == 2) {
+        // 16-bit values: a lane takes two neighbouring entries per load
+        // (4 B of destinations and 4 B of values; a0 is 64-aligned, and the
+        // arrays carry a 64-entry tail so the last pair is in bounds)
+        const uint32_t *kb2 = (const uint32_t *)kb;
+        const uint32_t *vb2 = (const uint32_t *)vb;
+        const uint32_t len2 = (len + 1) / 2, lo2 = lo / 2;
+        for (uint32_t base = w * 64; base < len2; base += nw * 64 * CA_U) {
+            uint32_t kp[CA_U], vp[CA_U];
+#pragma unroll
+            for (int u = 0; u < CA_U; ++u) {
+                const uint32_t q = base + u * nw * 64 + ln;
+                const uint32_t qc = q >= lo2 && q < len2 ? q : lo2;
+                kp[u] = __builtin_nontemporal_load(&kb2[qc]);
+                vp[u] = __builtin_nontemporal_load(&vb2[qc]);
+            }
+#pragma unroll
+            for (int u = 0; u < CA_U; ++u) {
+                const uint32_t i = 2 * (base + u * nw * 64 + ln);
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const long long fx = to_fixed(C::dec((uint16_t)(vp[u] >> (16 * h))) * dscale, shift);
+                    if (i + h >= lo && i + h < len && fx != 0)
+                        atomicAdd(&acc[(kp[u] >> (16 * h)) & 0xffffu], (unsigned long long)fx);
+                }
+            }
+        }
+    } else
     for (uint32_t base = w * 64; base < len; base += nw * 64 * CA_U) {
         uint16_t key[CA_U];
         typename C::store_t raw[CA_U];
