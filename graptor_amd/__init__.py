@@ -37,13 +37,13 @@ class EmapDesc(C.Structure):
     _fields_ = [("relax", C.c_int), ("prop", C.c_void_p), ("prev", C.c_void_p), ("src_strong", C.c_int),
                 ("dst_filter", C.c_int), ("dst_arg", C.c_uint32), ("record", C.c_int), ("direction", C.c_int),
                 ("out_kind", C.c_int), ("threshold_pct", C.c_float), ("contrib", C.c_void_p),
-                ("contrib_fmt", C.c_int), ("damping", C.c_float)]
+                ("contrib_fmt", C.c_int), ("damping", C.c_float), ("fusion", C.c_int)]
 
 
 class IterStats(C.Structure):
     _fields_ = [("nactv_in", C.c_uint32), ("nacte_in", C.c_uint64), ("nactv_out", C.c_uint32),
                 ("nacte_out", C.c_uint64), ("direction", C.c_int), ("out_kind", C.c_int),
-                ("edges_scanned", C.c_uint64), ("ms", C.c_double)]
+                ("edges_scanned", C.c_uint64), ("ms", C.c_double), ("fused", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -51,7 +51,7 @@ class IterStats(C.Structure):
 
 class RunOpts(C.Structure):
     _fields_ = [("direction", C.c_int), ("out_kind", C.c_int), ("threshold_pct", C.c_float),
-                ("max_iter", C.c_int), ("tol", C.c_double), ("trace", C.c_int)]
+                ("max_iter", C.c_int), ("tol", C.c_double), ("trace", C.c_int), ("fusion", C.c_int)]
 
 
 class RunInfo(C.Structure):
@@ -194,8 +194,13 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def opts(direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, max_iter=100, tol=1e-7, trace=False):
-    return RunOpts(direction, out_kind, threshold_pct, max_iter, tol, 1 if trace else 0)
+FUSION_OFF, FUSION_ON, FUSION_DEFAULT = range(3)
+
+
+def opts(direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, max_iter=100, tol=1e-7, trace=False, fusion=0):
+    """Run options.  fusion: 0 = the benchmark's reference default, -1 = off,
+    1 = Graptor's FUSION 1 rule, 2 = every sparse step fused."""
+    return RunOpts(direction, out_kind, threshold_pct, max_iter, tol, 1 if trace else 0, fusion)
 
 
 class Graph:
@@ -362,9 +367,9 @@ class Graph:
 
     def edgemap(self, relax, prop, frontier_in=None, frontier_out=None, dst_filter=DST_NONE, dst_arg=0,
                 record=RECORD_REDUCTION, direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, prev=None,
-                contrib=None, contrib_fmt=CONTRIB_F32, damping=0.85):
+                contrib=None, contrib_fmt=CONTRIB_F32, damping=0.85, fusion=FUSION_OFF):
         d = EmapDesc(relax, prop.h, prev.h if prev else None, 0, dst_filter, dst_arg, record, direction, out_kind,
-                     threshold_pct, contrib.h if contrib else None, contrib_fmt, damping)
+                     threshold_pct, contrib.h if contrib else None, contrib_fmt, damping, fusion)
         st = IterStats()
         _check(lib().gpr_edgemap(self.h, C.byref(d), frontier_in.h if frontier_in else None,
                                  frontier_out.h if frontier_out else None, C.byref(st)))

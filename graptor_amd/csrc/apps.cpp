@@ -22,6 +22,7 @@
 struct gpr_graph;
 namespace gpr {
 std::vector<gpr_iter_stats> &trace_of(gpr_graph *g);
+uint32_t graph_max_degree(const gpr_graph *g);
 void trace_clear(gpr_graph *g);
 double stream_time_ms(gpr_graph *g, bool start);
 }
@@ -81,6 +82,12 @@ struct BFSLVLv {
         check(gpr_policy_reset(GA.get()), "policy reset");
         info->edges += F.nActiveEdges();
         iter = 0;
+        // FUSION (BFSLVLv.C:28-40, 143-152, 246-250): off by default; 1 = on
+        // for low-degree graphs (cgraph.h:45-49) and after a dense step;
+        // 2 = always
+        const int fmode = o.fusion == 0 ? -1 : o.fusion;
+        bool enable_fusion = fmode == 2 || (fmode == 1 && (uint64_t)graph_max_degree(GA.get()) * (128 * 1024) < n);
+        bool fused = false;
         while (!F.isEmpty()) {
             A::frontier output;
             auto lazy = A::edgemap(GA,
@@ -88,15 +95,30 @@ struct BFSLVLv {
                                    A::filter(A::weak, A::src, F),
                                    A::filter(A::weak, A::dst, A::gt(a_level, (uint32_t)iter + 1)),
                                    A::relax(A::min_plus1(a_level)),
-                                   A::config(o.direction, o.threshold_pct, o.out_kind));
+                                   A::config(o.direction, o.threshold_pct, o.out_kind),
+                                   A::fusion_select(enable_fusion));
             lazy.materialize();
             record_step(GA.get(), o, lazy.stats(), info);
+            fused |= lazy.stats().fused != 0;
             info->edges += output.nActiveEdges();
             F.del();
             F = output;
             ++iter;
+            // api.h:150-153 default_threshold::is_sparse on the new frontier
+            if (fmode == 1 && (float)((uint64_t)F.nActiveVertices() + F.nActiveEdges()) > (float)GA.numEdges() / 20.f)
+                enable_fusion = true;
         }
         F.del();
+        if (fused) {
+            // a fused step leaves no frontiers to count: the edges of the
+            // reached vertices, from their levels
+            A::frontier all = A::frontier::all_true(GA);
+            A::frontier reached;
+            A::make_lazy_executor(GA).vertex_filter(all, reached, a_level, n + 1, true).materialize();
+            info->edges = reached.nActiveEdges();
+            reached.del();
+            all.del();
+        }
     }
 };
 
@@ -109,7 +131,12 @@ struct CCv {
     CCv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), m_IDs(GA, "IDs") {}
     ~CCv() { m_IDs.del(); }
 
-    // CCv.C:520-588
+    // CCv.C:520-588; FUSION (CCv.C:47-54, 562-566): 1 by default, i.e.
+    // api::fusion with default_fusion_select; 2 fuses every sparse step
+    int fusion_mode() const {
+        const int f = o.fusion == 0 ? 1 : o.fusion;
+        return f == 2 ? GPR_FUSION_ON : f == 1 ? GPR_FUSION_DEFAULT : GPR_FUSION_OFF;
+    }
     A::frontier emap_step(A::frontier &F, float threshold, gpr_run_info *info) {
         A::frontier output;
         int dir = o.direction;
@@ -119,7 +146,8 @@ struct CCv {
                                A::record(output, A::reduction, A::strong),
                                A::filter(A::weak, A::src, F),
                                A::filter(A::weak, A::dst, A::ne(m_IDs, 0)),
-                               A::relax(A::min_label(m_IDs)));
+                               A::relax(A::min_label(m_IDs)),
+                               A::fusion_mode(fusion_mode()));
         lazy.materialize();
         record_step(GA.get(), o, lazy.stats(), info);
         return output;

@@ -484,6 +484,34 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     const uint32_t *prev = desc->prev ? (const uint32_t *)desc->prev->d : nullptr;
     GPR_REQUIRE(in || dir != GPR_DIR_PUSH,
                 "edgemap: push over the implicit all-true frontier needs an explicit frontier");
+    // api::fusion: a sparse step that processes what it activates at once
+    // (fusion_select(true), or default_fusion_select: >= 4096 edges,
+    // api.h:227-248); the record and dst filter are implied by the relax
+    const bool fuse = dir == GPR_DIR_PUSH && in &&
+                      (desc->fusion == GPR_FUSION_ON || (desc->fusion == GPR_FUSION_DEFAULT && nacte >= 4096));
+    if (fuse) {
+        GPR_HIP(hipEventRecord(g->ev0, g->stream));
+        if (in_kind != GPR_FRONTIER_LIST) GPR_TRY(frontier_to_list(in));
+        // with the direction left to the engine, a level that turns dense
+        // ends the fused step and is returned, so the next step can pull
+        uint64_t dense_at = 0;
+        if (desc->direction == GPR_DIR_AUTO || desc->direction == GPR_DIR_POLICY) {
+            const float pct = desc->threshold_pct > 0.f ? desc->threshold_pct : 5.f;
+            dense_at = (uint64_t)((double)m / (100.0 / pct));
+        }
+        GPR_TRY(fused_push(g, desc->relax, (uint32_t *)desc->prop->d, in, out, dense_at, &s));
+        GPR_HIP(hipEventRecord(g->ev1, g->stream));
+        GPR_HIP(hipEventSynchronize(g->ev1));
+        float ms = 0.f;
+        GPR_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        if (use_pol) policy_after_step(g, nacte, 0);
+        s.direction = GPR_DIR_PUSH;
+        s.out_kind = out ? out->kind : GPR_FRONTIER_EMPTY;
+        s.fused = 1;
+        s.ms = ms;
+        if (st) *st = s;
+        return GPR_OK;
+    }
     GPR_HIP(hipEventRecord(g->ev0, g->stream));
     // representation changes first: they use the counters themselves
     const uint64_t *in_bits = nullptr;

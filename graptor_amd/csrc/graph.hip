@@ -135,6 +135,8 @@ static void graph_free(gpr_graph *g) {
     dev_free(g->prs);
     dev_free(g->pol_params);
     cold_free(g);
+    dev_free(g->fbuf);
+    dev_free(g->fctl);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->rev0) hipEventDestroy(g->rev0);
@@ -197,6 +199,8 @@ int graph_new_device(uint32_t n, uint64_t m, gpr_graph **out) {
 }
 
 void graph_release(gpr_graph *g) { graph_free(g); }
+
+uint32_t graph_max_degree(const gpr_graph *g) { return g->max_deg; }
 
 // Install a new (relabelled, symmetric) CSR in place of the current one and
 // rebuild the degree array and pull partitions; scratch stays.

@@ -65,6 +65,8 @@ struct Tunables {
     double pull_blocks_per_cu = 8;            // pull grid = CUs * this
     double push_blocks_per_cu = 32;           // push grid cap = CUs * this
     double pr_split_bytes = 0;                // FP32 bytes of pulled sources (0: all binned; >= 4n: no split)
+    double fusion_spin_limit = 1 << 22;       // fused push: polls without progress before it gives up
+    double fusion_blocks = 0;                 // fused push: cooperative grid size (0: one per CU)
 };
 Tunables &tun();
 int num_cus();
@@ -187,6 +189,9 @@ struct gpr_graph {
     double pol_unvisited = 0;
     uint32_t max_deg = 0;
     gpr::ColdLayout *cold = nullptr;   // PageRank cold-edge layout (built on first use)
+    void *fbuf = nullptr;              // fused push: level lists and second bitmap (built on first use)
+    size_t fbuf_bytes = 0;
+    void *fctl = nullptr;              // fused push: control block
 };
 
 struct gpr_prop {
@@ -234,6 +239,11 @@ struct ProfScope {
     }
 };
 void prof_flush(gpr_graph *g);
+
+// Fused sparse push (fusion.hip): process the frontier and everything it
+// activates in one launch; `in` must be a list.
+int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_frontier *out, uint64_t dense_at,
+               gpr_iter_stats *s);
 
 // PageRank cold-edge layout (pr_cold.hip)
 int cold_prepare(gpr_graph *g, uint32_t from);   // (re)build for this split point

@@ -201,6 +201,8 @@ static double *tunable(const char *name) {
     if (!strcmp(name, "pull_blocks_per_cu")) return &t.pull_blocks_per_cu;
     if (!strcmp(name, "push_blocks_per_cu")) return &t.push_blocks_per_cu;
     if (!strcmp(name, "pr_split_bytes")) return &t.pr_split_bytes;
+    if (!strcmp(name, "fusion_spin_limit")) return &t.fusion_spin_limit;
+    if (!strcmp(name, "fusion_blocks")) return &t.fusion_blocks;
     return nullptr;
 }
 

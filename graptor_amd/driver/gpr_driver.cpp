@@ -22,6 +22,8 @@
 //   -reorder           relabel by decreasing degree (GraphVEBOGraptor's role)
 //   -dir D             auto | push | pull | policy  (auto = m/20 threshold)
 //   -threshold T       frac_threshold(T) percent (5)
+//   -fusion F          fused sparse steps: default|off|1|2 (the benchmarks'
+//                      FUSION setting, BFSLVLv.C:28-40, CCv.C:47-54)
 //   -policy F          network weights (JSON with a "params" array) for -dir policy
 //   -pr:stages L       PR_vary stages fmt:tol,... with fmt f32|bf16|f16|cfp16|bf16t
 //                      (default cfp16:1e-3,f32:1e-7, PR_vary.C:41-43 VARIANT 4620)
@@ -102,7 +104,7 @@ void usage() {
     std::cerr << "usage: gpr_driver -app bfslvl|cc|pr|pr_vary [-b] [-s] [-rounds R] [-start V]\n"
                  "                  [-pr:maxiter K] [-pr:outfile F] [-cc:outfile F] [-itimes]\n"
                  "                  [-rmat S [-ef E] [-seed X]] [-reorder] [-dir auto|push|pull|policy]\n"
-                 "                  [-threshold T] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
+                 "                  [-threshold T] [-fusion default|off|1|2] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
                  "                  <inFile>\n";
 }
 
@@ -346,6 +348,9 @@ int run(const Args &a) {
     o.max_iter = (int)a.num("-pr:maxiter", 100);
     o.tol = 1e-7;
     o.trace = a.has("-itimes") ? 1 : 0;
+    const std::string fu = a.str("-fusion", "default");
+    o.fusion = fu == "default" ? 0 : fu == "off" ? -1 : fu == "1" ? 1 : fu == "2" ? 2 : -2;
+    if (o.fusion == -2) throw std::runtime_error("unknown -fusion " + fu);
     if (a.has("-pr:infile")) throw std::runtime_error("-pr:infile is not supported: runs start from x = 1/n");
 
     CALL(gpr_init((int)a.num("-device", 0)));

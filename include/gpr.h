@@ -173,7 +173,16 @@ typedef struct {
     gpr_prop *contrib;
     int contrib_fmt;
     float damping;        /* 0.85 when 0 */
+    /* api::fusion with api::fusion_select (api/fusion.h:17-140, api.h:227-248):
+     * a push step processes the vertices it activates at once, in the same
+     * launch, until none is left; its output frontier is then empty.  For
+     * GPR_RELAX_MIN_PLUS1 / GPR_RELAX_MIN; the dst filter is implied by the
+     * relax.  GPR_FUSION_OFF, _ON (fusion_select(true)), _DEFAULT
+     * (default_fusion_select: when the frontier has >= 4096 edges). */
+    int fusion;
 } gpr_emap_desc;
+
+enum { GPR_FUSION_OFF = 0, GPR_FUSION_ON = 1, GPR_FUSION_DEFAULT = 2 };
 
 typedef struct {
     uint32_t nactv_in;
@@ -182,8 +191,10 @@ typedef struct {
     uint64_t nacte_out;
     int direction;        /* GPR_DIR_PUSH or GPR_DIR_PULL as executed */
     int out_kind;         /* GPR_FRONTIER_* produced */
-    uint64_t edges_scanned; /* edges examined (upper bound for pull) */
+    uint64_t edges_scanned; /* edges examined (upper bound for pull; every
+                               relaxation of a fused step, repeats included) */
     double ms;            /* device time of the step */
+    int fused;            /* 1: a fused push step (api::fusion) */
 } gpr_iter_stats;
 
 /* One edgemap step over frontier `in`, recording into `out` (which may be
@@ -229,6 +240,12 @@ typedef struct {
     int max_iter;         /* PR: -pr:maxiter (default 100) */
     double tol;           /* PR: 1e-7 */
     int trace;            /* record per-iteration stats (-itimes) */
+    /* fused sparse steps (the benchmarks' FUSION setting, BFSLVLv.C:28-40,
+     * CCv.C:47-54): 0 = the reference's default for the benchmark (BFSLVLv
+     * off, CCv FUSION 1), -1 = off, 1 = FUSION 1 (BFSLVLv: on for
+     * low-degree graphs and after a dense step; CCv: default_fusion_select),
+     * 2 = FUSION 2 (every sparse step) */
+    int fusion;
 } gpr_run_opts;
 
 typedef struct {
@@ -325,7 +342,12 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
  *                       edge.  Default 0
  *   pull_serial_max     pull rows up to this in-degree run lane-serially
  *   pull_blocks_per_cu  pull grid size per compute unit
- *   push_blocks_per_cu  push grid cap per compute unit */
+ *   push_blocks_per_cu  push grid cap per compute unit
+ *   fusion_spin_limit   fused push: polls of an idle wave without any wave
+ *                       claiming work, or of a reserved queue slot, before
+ *                       the step fails with GPR_E_HIP (default 2^22)
+ *   fusion_blocks       fused push: blocks of 1024 threads in its
+ *                       cooperative grid (0: one per compute unit) */
 int gpr_tune_set(const char *name, double value);
 int gpr_tune_get(const char *name, double *value);
 

@@ -260,6 +260,18 @@ inline config_arg config(int dir, float pct, int out_kind) {
     return c;
 }
 
+// api::fusion / api::fusion_select (api/fusion.h:104-117, api.h:227-248): a
+// sparse step that processes the vertices it activates at once.  The
+// reference's fusion lambda decides per destination; for the min relaxes it
+// is "process if changed", which the engine applies, so here the argument
+// only selects when a step fuses.
+struct fusion_arg {
+    int mode;
+};
+inline fusion_arg fusion() { return {GPR_FUSION_DEFAULT}; }   // default_fusion_select
+inline fusion_arg fusion_select(bool enabled) { return {enabled ? GPR_FUSION_ON : GPR_FUSION_OFF}; }
+inline fusion_arg fusion_mode(int mode) { return {mode}; }
+
 // ---- lazy executor (vertex ops run at materialize) ---------------------------
 struct vop {
     int op;
@@ -355,6 +367,7 @@ inline void apply(gpr_emap_desc &d, frontier *&, frontier *&, const config_arg &
     d.threshold_pct = a.pct;
     d.out_kind = a.out_kind;
 }
+inline void apply(gpr_emap_desc &d, frontier *&, frontier *&, const fusion_arg &a) { d.fusion = a.mode; }
 } // namespace detail
 
 // api::edgemap: one step, executed at once; the returned executor carries

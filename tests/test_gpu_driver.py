@@ -112,3 +112,31 @@ def test_driver_rmat_generation():
     index, nbrs = O.rmat_graph(12, 8)
     assert "graph: n=%d m=%d symmetric" % (len(index) - 1, len(nbrs)) in err
     assert "Unique values: %d" % len(np.unique(O.cc_labels(index, nbrs))) in out
+
+
+@pytest.mark.parametrize("app", ["bfslvl", "cc"])
+@pytest.mark.parametrize("fusion", ["off", "1", "2"])
+def test_driver_fusion(tmp_path, app, fusion):
+    """-fusion selects the benchmarks' FUSION setting; on a grid (high
+    diameter, sparse steps) the fused runs give the same answers."""
+    index, nbrs = O.grid_graph(60, 50, 0.0, seed=2)
+    path = str(tmp_path / "grid.b2")
+    O.b2_write(path, index, nbrs)
+    outfile = str(tmp_path / "cc.txt")
+    extra = ["-cc:outfile", outfile] if app == "cc" else ["-start", "0"]
+    out, _ = drive("-app", app, "-b", "-s", "-rounds", "2", "-fusion", fusion, *extra, path)
+    running_times(out, 2)
+    if app == "cc":
+        assert np.array_equal(np.loadtxt(outfile, dtype=np.int64), O.cc_labels(index, nbrs).astype(np.int64))
+        assert "Neighbour check: PASS" in _
+    else:
+        lv = O.bfs_levels(index, nbrs, 0)
+        assert re.search(r"Longest path from 0 \(original: 0\) : %d " % int(lv.max()), out), out
+        assert out.count("not activated in time: 0 PASS") == 2
+
+
+def test_driver_fusion_rejects_unknown_setting(graph):
+    path, _, _ = graph
+    r = subprocess.run([DRIVER, "-app", "cc", "-b", "-s", "-fusion", "3", path], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "unknown -fusion 3" in (r.stdout + r.stderr)
