@@ -103,32 +103,36 @@ struct PrScalars {
 // degree-ordered graph, whose gathers would each cost a cache line (all
 // sources when from = 0).  Cold edges are grouped into tiles of consecutive
 // sources (<= COLD_TILE_V sources, <= COLD_TILE_E edges) and bins of
-// COLD_BIN destinations.  The write kernel reads each tile's entries in
-// (bin, destination) order and writes each value to its slot in the
-// bin-major value array, where a bin's entries are contiguous (its runs, one
-// per tile, each sorted by destination); the accumulation kernel streams
-// each bin into LDS.  Bins with more than COLD_CHUNK_E entries are split into
-// chunks whose integer partials a third kernel adds up.
+// COLD_BIN destinations; the edges of one source into one bin form a pair,
+// which carries one value.  The write kernel reads each tile's pairs in
+// (bin, source) order and writes each pair's value to its slot in the
+// bin-major value array, where a bin's pairs are contiguous (its runs, one
+// per tile); the accumulation kernel streams each bin's entries (one per
+// edge, grouped by pair) into LDS.  Bins with more than COLD_CHUNK_E entries
+// are split into chunks whose integer partials a third kernel adds up.
 constexpr uint32_t COLD_BIN = 16384;
 constexpr uint32_t COLD_TILE_V = 32768;
-constexpr uint64_t COLD_TILE_E = 262144;
+constexpr uint64_t COLD_TILE_E = 1ull << 24;
+constexpr uint64_t COLD_WCHUNK = 262144;   // pairs per write chunk
 constexpr uint64_t COLD_CHUNK_E = 1u << 20;
 constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
 struct ColdLayout {
     uint32_t from = 0;              // first cold source
     uint32_t ntiles = 0, nbins = 0;
     uint64_t ncold = 0;             // cold edges (entries)
+    uint64_t npairs = 0;            // distinct (source, bin) pairs over the cold edges
     uint32_t *tile_lo = nullptr;    // ntiles+1 source boundaries
-    uint32_t *delta = nullptr;      // ntiles x nbins: bin-major slot - tile-major position of run (t,b)
-    uint32_t *wsrc = nullptr;       // per entry, tile-major: bin << 15 | (source - tile_lo[t])
-    uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN
-    uint32_t *vals = nullptr;       // per entry, bin-major: the staged contribution (4 or 2 bytes)
+    uint32_t *delta = nullptr;      // ntiles x nbins: bin-major pair slot - tile-major pair position of cell (t,b)
+    uint32_t *wsrc = nullptr;       // per pair, tile-major: bin << 15 | (source - tile_lo[t])
+    uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN, bit 15 = first entry of a pair
+    uint32_t *bbase = nullptr;      // per 64 bin-major entries: the pair of the first entry
+    uint32_t *vals = nullptr;       // per pair, bin-major: the staged contribution (4 or 2 bytes)
     uint32_t nwchunks = 0, nachunks = 0, nmulti = 0;
-    uint32_t *wchunk = nullptr;     // per write chunk: tile, first entry, end entry (tile-major), 0
+    uint32_t *wchunk = nullptr;     // per write chunk: tile, first pair, end pair (tile-major), first chunk of tile
     uint32_t *achunk = nullptr;     // per accumulation chunk: bin, first entry, end entry (bin-major), slot
     uint32_t *multi = nullptr;      // per bin in several chunks: bin, first slot, chunks, 0
     long long *partials = nullptr;  // COLD_BIN fixed-point sums per slot
-    double *mass = nullptr;         // per write chunk: sum of |contribution| over its entries
+    double *mass = nullptr;         // per write chunk: sum over its tile's edges of |contribution| (first chunk)
 };
 
 struct Profile {

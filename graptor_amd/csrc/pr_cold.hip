@@ -1,4 +1,5 @@
-// pr_cold.hip -- PageRank edges from "cold" sources, by propagation blocking.
+// pr_cold.hip -- PageRank edges from "cold" sources, by propagation blocking
+// with one value per (source, destination bin) pair.
 //
 // A pull gathers contrib[s] for every in-edge (s, d).  After degree-ordered
 // relabelling the contributions of the first sources stay in the L2, but a
@@ -8,35 +9,46 @@
 // from = 0) are therefore summed by streaming instead:
 //
 //   k_cold_write    per chunk of a tile of consecutive sources: the tile's
-//                   contributions are staged in LDS; its entries are read in
-//                   (bin, destination) order (a coalesced 4 B word: bin and
-//                   source offset) and each value is written to its slot in
-//                   the bin-major value array, where the tile's entries of
-//                   one bin form a contiguous run; the chunk's
-//                   |contribution| mass is summed.
+//                   contributions are staged in LDS; its pairs are read in
+//                   (bin, source) order (a coalesced 4 B word: bin and
+//                   source offset) and each pair's value is written once to
+//                   its slot in the bin-major value array, where the tile's
+//                   pairs of one bin form a contiguous run.  The first chunk
+//                   of a tile also sums |contribution| x out-degree over the
+//                   tile's sources (the mass of its edges).
 //   k_cold_accum    per chunk of a bin of COLD_BIN destinations: the bin's
-//                   entries (values and 2 B destinations) are contiguous and
-//                   streamed; equal destinations next to each other (runs
-//                   are sorted by destination) are combined across the wave,
-//                   and the sums go into LDS as 64-bit fixed-point integers,
-//                   so the result does not depend on the order of the
-//                   atomics.  A bin in one chunk updates y directly; the
-//                   chunks of a heavy bin (the hub destinations) write
-//                   integer partials that
+//                   entries (2 B destinations, one per edge, grouped by
+//                   pair) are streamed; bit 15 marks the first entry of a
+//                   pair, so an entry's pair is the pair of its 64-entry
+//                   block's first entry plus a ballot prefix count; a wave
+//                   loads the block's next 64 pair values beside its
+//                   destinations and hands each lane its value by shuffle.  The sums go into
+//                   LDS as 64-bit fixed-point integers, so the result does
+//                   not depend on the order of the atomics.  A bin in one
+//                   chunk updates y directly; the chunks of a heavy bin (the
+//                   hub destinations) write integer partials that
 //   k_cold_combine  adds up, per destination.
 //
+// After degree relabelling a source's edges fall into few bins relative to
+// its degree (RMAT: 0.10 / 0.16 / 0.22 pairs per edge at scales 20 / 22 /
+// 24), so a value crosses HBM once per pair instead of once per edge; the
+// per-edge traffic is the 2 B destination.  This is the "partition-centric"
+// form of propagation blocking (Lakhotia, Kannan and Prasanna, USENIX ATC
+// 2018).
+//
 // The fixed-point unit 2^E is chosen per iteration from the total mass M (a
-// deterministic FP64 sum of the write chunks' masses): every partial sum is
-// at most M <= 2^61 units, and a contribution loses less than one unit.
-// Write chunks go to the XCDs in contiguous ranges of tiles, so that the
-// runs of neighbouring tiles, which share cache lines at their ends, are
-// written through the same L2.
+// deterministic FP64 sum of the tiles' masses): every partial sum is at most
+// M <= 2^61 units, and a contribution loses less than one unit.  Write
+// chunks go to the XCDs in contiguous ranges of tiles, so that the runs of
+// neighbouring tiles, which share cache lines at their ends, are written
+// through the same L2.
 //
 // The layout is static for a graph and split point: built once on the
-// device (count per (tile, bin), a radix sort of the entries by
-// (tile, bin, destination)), planned on the host (run offsets, chunks) and
-// kept until the graph's CSR changes.  Entries are the CSR edges s -> d with
-// s >= from, so directed graphs work the same way.
+// device (edge counts per (tile, bin), a stable radix sort of the edges by
+// (tile, bin) that keeps each source's edges together, pair flags and their
+// prefix sum, pair counts per cell), planned on the host (run offsets,
+// chunks) and kept until the graph's CSR changes.  Entries are the CSR edges
+// s -> d with s >= from, so directed graphs work the same way.
 #include "internal.hpp"
 #include "device.hpp"
 #include "pr_codec.hpp"
@@ -70,13 +82,13 @@ __device__ __forceinline__ uint32_t cold_row_of(const uint64_t *index, uint32_t 
 // Over the cold edges in tiles of CK_EDGES consecutive edges (balanced
 // whatever the degrees; the rows of a tile found through LDS-staged offsets
 // when they fit).  mode 0: count entries per (tile, bin).  mode 1: write
-// the sort key (tile*nbins + bin) << 14 | (destination in bin) and the
-// source offset in the tile at the edge's position.
+// the sort key tile*nbins + bin and the word (source offset in the tile) << 14
+// | (destination in bin) at the edge's position.
 constexpr int CK_EDGES = 4096;
 __global__ void __launch_bounds__(256)
 k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
-            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt, uint64_t *keys,
-            uint16_t *vals) {
+            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt, uint32_t *keys,
+            uint32_t *vals) {
     __shared__ uint64_t sidx[CK_EDGES + 1];
     __shared__ uint32_t s_r0, s_cnt, s_t0;
     const uint32_t first = tile_lo[0], last = tile_lo[ntiles];
@@ -113,28 +125,49 @@ k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__rest
             while (t + 1 < ntiles && tile_lo[t + 1] <= u) ++t;
             const uint32_t v = nbrs[e];
             const uint32_t b = v / COLD_BIN;
-            const uint64_t cell = (uint64_t)t * nbins + b;
+            const uint32_t cell = t * nbins + b;   // < 2^32: checked by the host
             if (mode == 0) {
                 atomicAdd(&cnt[cell], 1u);
             } else {
-                keys[e - E0] = (cell << 14) | (v - b * COLD_BIN);
-                vals[e - E0] = (uint16_t)(u - tile_lo[t]);
+                keys[e - E0] = cell;
+                vals[e - E0] = ((u - tile_lo[t]) << 14) | (v - b * COLD_BIN);
             }
         }
         __syncthreads();
     }
 }
 
-// Sorted entry p (tile-major): the write kernel's word bin << 15 | source
-// offset, and the destination in bin at the entry's bin-major slot.
-__global__ void k_cold_finish(const uint64_t *__restrict__ keys, const uint16_t *__restrict__ so, uint64_t count,
-                              uint32_t nbins, const uint32_t *__restrict__ delta, uint32_t *wsrc, uint16_t *cdst) {
+// Sorted entry p (tile-major, a source's edges of one bin together): 1 when
+// it starts a pair (the first entry of its cell, or a new source), counted
+// per cell.
+__global__ void k_cold_flags(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ words, uint64_t count,
+                             uint32_t *flag, uint32_t *pcnt) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < count; p += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t k = keys[p];
-        const uint64_t cell = k >> 14;
-        const uint32_t b = (uint32_t)(cell % nbins);
-        wsrc[p] = (b << 15) | so[p];
-        cdst[(uint32_t)p + delta[cell]] = (uint16_t)(k & (COLD_BIN - 1));
+        const uint32_t k = keys[p];
+        const bool f = p == 0 || k != keys[p - 1] || (words[p] >> 14) != (words[p - 1] >> 14);
+        flag[p] = f ? 1u : 0u;
+        if (f) atomicAdd(&pcnt[k], 1u);
+    }
+}
+
+// Sorted entry p, with incl = inclusive prefix sum of the flags: its
+// destination (and pair flag) at its bin-major slot p + edelta[cell]; a
+// pair's write word at the pair's tile-major position incl[p] - 1; and for
+// an entry on a 64-entry boundary of the bin-major order, the block's pair
+// base (the bin-major pair of that entry).
+__global__ void k_cold_finish(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ words,
+                              const uint32_t *__restrict__ incl, uint64_t count, uint32_t nbins,
+                              const uint32_t *__restrict__ edelta, const uint32_t *__restrict__ pdelta,
+                              uint32_t *wsrc, uint16_t *cdst, uint32_t *bbase) {
+    for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < count; p += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t cell = keys[p], w = words[p];
+        const uint32_t b = cell % nbins;
+        const uint32_t q = incl[p] - 1;   // tile-major pair
+        const uint32_t f = incl[p] - (p ? incl[p - 1] : 0u);
+        const uint32_t E = (uint32_t)p + edelta[cell];   // modulo 2^32, as the offsets
+        cdst[E] = (uint16_t)((w & (COLD_BIN - 1)) | (f << 15));
+        if (f) wsrc[q] = (b << 15) | (w >> 14);
+        if ((E & 63u) == 0) bbase[E >> 6] = q + pdelta[cell];
     }
 }
 
@@ -146,6 +179,7 @@ void cold_free(gpr_graph *g) {
     dev_free(L->delta);
     dev_free(L->wsrc);
     dev_free(L->cdst);
+    dev_free(L->bbase);
     dev_free(L->vals);
     dev_free(L->mass);
     dev_free(L->wchunk);
@@ -186,6 +220,28 @@ static uint32_t bits_for(uint64_t v) {
     return b;
 }
 
+// Run offsets of a cell-major count table (cells t*nbins + b): the
+// bin-major position of cell (t,b) minus its tile-major position, modulo
+// 2^32 (both positions are < 2^32); bstart gets each bin's first position.
+static void plan_runs(const std::vector<uint32_t> &cnt, uint32_t ntiles, uint32_t nbins, std::vector<uint32_t> &delta,
+                      std::vector<uint64_t> &bstart) {
+    std::vector<uint64_t> bsize(nbins, 0);
+    for (uint32_t t = 0; t < ntiles; ++t)
+        for (uint32_t b = 0; b < nbins; ++b) bsize[b] += cnt[(size_t)t * nbins + b];
+    bstart.assign(nbins + 1, 0);
+    for (uint32_t b = 0; b < nbins; ++b) bstart[b + 1] = bstart[b] + bsize[b];
+    std::vector<uint64_t> bpos(bstart.begin(), bstart.end() - 1);   // next slot of each bin
+    delta.resize((size_t)ntiles * nbins);
+    uint64_t pos = 0;
+    for (uint32_t t = 0; t < ntiles; ++t)
+        for (uint32_t b = 0; b < nbins; ++b) {
+            const size_t cell = (size_t)t * nbins + b;
+            delta[cell] = (uint32_t)(bpos[b] - pos);
+            bpos[b] += cnt[cell];
+            pos += cnt[cell];
+        }
+}
+
 int cold_prepare(gpr_graph *g, uint32_t from) {
     if (g->cold && g->cold->from == from) return GPR_OK;
     cold_free(g);
@@ -211,84 +267,61 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     idx.shrink_to_fit();
     const uint32_t ntiles = (uint32_t)lo.size() - 1;
     const size_t cells = (size_t)ntiles * nbins;
-    uint32_t *cnt = nullptr;
-    uint64_t *k0 = nullptr, *k1 = nullptr;
-    uint16_t *v0 = nullptr, *v1 = nullptr;
+    if (cells >= 0xffffffffull) return GPR_OK;   // cell keys are 32-bit: the pull handles every edge
+    uint32_t *cnt = nullptr, *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *edelta = nullptr;
     void *tmp = nullptr;
     int r = GPR_OK;
-    std::vector<uint32_t> hc, delta(cells);
+    std::vector<uint32_t> ecnt, pcnt, edel, pdel;
     std::vector<uint32_t> wch, ach, multi;   // plans, 4 words per item
     uint32_t nslots = 0;
+    uint64_t npairs = 0;
+    const unsigned pg = grid_for(ncold, CK_EDGES, 65536);
+    const unsigned eg = grid_for(ncold, 256, 65536);
     do {
         if ((r = dev_alloc((void **)&L->tile_lo, sizeof(uint32_t) * lo.size(), "cold tiles"))) break;
         if ((r = dev_alloc((void **)&cnt, sizeof(uint32_t) * cells, "cold counts"))) break;
         if (hipMemcpyAsync(L->tile_lo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, g->stream) !=
             hipSuccess) { r = GPR_E_HIP; break; }
+        // ---- edges per cell: the bin-major entry positions
         if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        const unsigned pg = grid_for(ncold, CK_EDGES, 65536);
         hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 0, g->index, g->nbrs, L->tile_lo, ntiles,
-                           nbins, cnt, (uint64_t *)nullptr, (uint16_t *)nullptr);
-        hc.resize(cells);
-        if (hipMemcpyAsync(hc.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                           nbins, cnt, (uint32_t *)nullptr, (uint32_t *)nullptr);
+        ecnt.resize(cells);
+        if (hipMemcpyAsync(ecnt.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) !=
+                hipSuccess ||
             hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        dev_free(cnt);
-        cnt = nullptr;
-        // ---- host plan: tile-major run positions, bin-major slots, chunks
-        {
-            std::vector<uint64_t> bsize(nbins, 0), bstart(nbins + 1, 0);
-            for (uint32_t t = 0; t < ntiles; ++t)
-                for (uint32_t b = 0; b < nbins; ++b) bsize[b] += hc[(size_t)t * nbins + b];
-            for (uint32_t b = 0; b < nbins; ++b) bstart[b + 1] = bstart[b] + bsize[b];
-            std::vector<uint64_t> bpos(bstart.begin(), bstart.end() - 1);   // next slot of each bin
-            uint64_t pos = 0;
-            for (uint32_t t = 0; t < ntiles; ++t) {
-                const uint64_t t0 = pos;
-                for (uint32_t b = 0; b < nbins; ++b) {
-                    const uint32_t c = hc[(size_t)t * nbins + b];
-                    delta[(size_t)t * nbins + b] = (uint32_t)(bpos[b] - pos);   // modulo 2^32
-                    bpos[b] += c;
-                    pos += c;
-                }
-                for (uint64_t p = t0; p < pos; p += COLD_TILE_E) {
-                    wch.push_back(t);
-                    wch.push_back((uint32_t)p);
-                    wch.push_back((uint32_t)std::min<uint64_t>(pos, p + COLD_TILE_E));
-                    wch.push_back(0);
-                }
+        std::vector<uint64_t> ebstart;
+        plan_runs(ecnt, ntiles, nbins, edel, ebstart);
+        ecnt.clear();
+        ecnt.shrink_to_fit();
+        for (uint32_t b = 0; b < nbins; ++b) {
+            const uint64_t e0 = ebstart[b], e1 = ebstart[b + 1];
+            const uint32_t k = e1 > e0 ? (uint32_t)((e1 - e0 + COLD_CHUNK_E - 1) / COLD_CHUNK_E) : 1u;
+            for (uint32_t i = 0; i < k; ++i) {
+                ach.push_back(b);
+                ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)i * COLD_CHUNK_E));
+                ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)(i + 1) * COLD_CHUNK_E));
+                ach.push_back(k == 1 ? COLD_DIRECT : nslots + i);
             }
-            for (uint32_t b = 0; b < nbins; ++b) {
-                const uint64_t e0 = bstart[b], e1 = bstart[b + 1];
-                const uint32_t k = e1 > e0 ? (uint32_t)((e1 - e0 + COLD_CHUNK_E - 1) / COLD_CHUNK_E) : 1u;
-                for (uint32_t i = 0; i < k; ++i) {
-                    ach.push_back(b);
-                    ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)i * COLD_CHUNK_E));
-                    ach.push_back((uint32_t)std::min<uint64_t>(e1, e0 + (uint64_t)(i + 1) * COLD_CHUNK_E));
-                    ach.push_back(k == 1 ? COLD_DIRECT : nslots + i);
-                }
-                if (k > 1) {
-                    multi.push_back(b);
-                    multi.push_back(nslots);
-                    multi.push_back(k);
-                    multi.push_back(0);
-                    nslots += k;
-                }
+            if (k > 1) {
+                multi.push_back(b);
+                multi.push_back(nslots);
+                multi.push_back(k);
+                multi.push_back(0);
+                nslots += k;
             }
         }
-        hc.clear();
-        hc.shrink_to_fit();
-        if ((r = dev_alloc((void **)&L->delta, sizeof(uint32_t) * cells, "cold run offsets"))) break;
-        if (hipMemcpyAsync(L->delta, delta.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
-            hipSuccess) { r = GPR_E_HIP; break; }
-        // ---- entries sorted by (tile, bin, destination in bin)
-        if ((r = dev_alloc((void **)&k0, 8 * ncold, "cold sort keys"))) break;
-        if ((r = dev_alloc((void **)&k1, 8 * ncold, "cold sort keys"))) break;
-        if ((r = dev_alloc((void **)&v0, 2 * ncold, "cold sources"))) break;
-        if ((r = dev_alloc((void **)&v1, 2 * ncold, "cold sources"))) break;
+        // ---- edges sorted by (tile, bin), stable: a source's edges of one
+        // bin stay together
+        if ((r = dev_alloc((void **)&k0, 4 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&k1, 4 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&v0, 4 * ncold, "cold sort words"))) break;
+        if ((r = dev_alloc((void **)&v1, 4 * ncold, "cold sort words"))) break;
         hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 1, g->index, g->nbrs, L->tile_lo, ntiles,
                            nbins, (uint32_t *)nullptr, k0, v0);
-        rocprim::double_buffer<uint64_t> kb(k0, k1);
-        rocprim::double_buffer<uint16_t> vb(v0, v1);
-        const unsigned end_bit = 14 + bits_for(cells);
+        rocprim::double_buffer<uint32_t> kb(k0, k1);
+        rocprim::double_buffer<uint32_t> vb(v0, v1);
+        const unsigned end_bit = bits_for(cells);
         size_t tsz = 0;
         if (rocprim::radix_sort_pairs(nullptr, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
             r = GPR_E_HIP;
@@ -301,18 +334,65 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         }
         dev_free(tmp);
         tmp = nullptr;
-        if ((r = dev_alloc((void **)&L->wsrc, 4 * ncold, "cold write words"))) break;
+        // ---- pair flags (into the sort's spare key buffer), pairs per cell,
+        // and the flags' prefix sum (into the spare word buffer)
+        uint32_t *flag = kb.alternate(), *incl = vb.alternate();
+        if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        hipLaunchKernelGGL(k_cold_flags, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), ncold, flag,
+                           cnt);
+        tsz = 0;
+        if (rocprim::inclusive_scan(nullptr, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        if ((r = dev_alloc(&tmp, tsz, "cold scan workspace"))) break;
+        if (rocprim::inclusive_scan(tmp, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(tmp);
+        tmp = nullptr;
+        pcnt.resize(cells);
+        if (hipMemcpyAsync(pcnt.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(cnt);
+        cnt = nullptr;
+        // ---- pair plan: bin-major pair slots, write chunks per tile
+        {
+            std::vector<uint64_t> pbstart;
+            plan_runs(pcnt, ntiles, nbins, pdel, pbstart);
+            npairs = pbstart[nbins];
+            uint64_t pos = 0;
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                const uint64_t t0 = pos;
+                for (uint32_t b = 0; b < nbins; ++b) pos += pcnt[(size_t)t * nbins + b];
+                for (uint64_t q = t0; q < pos; q += COLD_WCHUNK) {
+                    wch.push_back(t);
+                    wch.push_back((uint32_t)q);
+                    wch.push_back((uint32_t)std::min<uint64_t>(pos, q + COLD_WCHUNK));
+                    wch.push_back(q == t0 ? 1u : 0u);
+                }
+            }
+        }
+        pcnt.clear();
+        pcnt.shrink_to_fit();
+        if ((r = dev_alloc((void **)&L->delta, sizeof(uint32_t) * cells, "cold pair run offsets"))) break;
+        if ((r = dev_alloc((void **)&edelta, sizeof(uint32_t) * cells, "cold entry run offsets"))) break;
+        if (hipMemcpyAsync(L->delta, pdel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(edelta, edel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
+                hipSuccess) { r = GPR_E_HIP; break; }
+        if ((r = dev_alloc((void **)&L->wsrc, 4 * npairs, "cold write words"))) break;
         if ((r = dev_alloc((void **)&L->cdst, 2 * (ncold + 64), "cold destinations"))) break;
-        hipLaunchKernelGGL(k_cold_finish, dim3(grid_for(ncold, 256, 65536)), dim3(256), 0, g->stream, kb.current(),
-                           vb.current(), ncold, nbins, L->delta, L->wsrc, L->cdst);
+        if ((r = dev_alloc((void **)&L->bbase, 4 * (ncold / 64 + 2), "cold pair bases"))) break;
+        if (hipMemsetAsync(L->cdst, 0, 2 * (ncold + 64), g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        hipLaunchKernelGGL(k_cold_finish, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), incl, ncold,
+                           nbins, edelta, L->delta, L->wsrc, L->cdst, L->bbase);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
         dev_free(k0);
         dev_free(k1);
         dev_free(v0);
         dev_free(v1);
-        k0 = k1 = nullptr;
-        v0 = v1 = nullptr;
-        if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * (ncold + 64), "cold values"))) break;
+        dev_free(edelta);
+        k0 = k1 = v0 = v1 = edelta = nullptr;
+        if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * (npairs + 64), "cold values"))) break;
         L->nwchunks = (uint32_t)(wch.size() / 4);
         L->nachunks = (uint32_t)(ach.size() / 4);
         L->nmulti = (uint32_t)(multi.size() / 4);
@@ -336,6 +416,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     dev_free(k1);
     dev_free(v0);
     dev_free(v1);
+    dev_free(edelta);
     dev_free(tmp);
     if (r != GPR_OK) {
         if (r == GPR_E_HIP) set_error("PageRank cold-edge layout build failed");
@@ -345,6 +426,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     L->ntiles = ntiles;
     L->nbins = nbins;
     L->ncold = ncold;
+    L->npairs = npairs;
     return GPR_OK;
 }
 
@@ -360,15 +442,13 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t count) {
     return x * q + min(x, r) + k;
 }
 
-// The copy loop of one write chunk: entries [p0, p1) of the tile, values
+// The copy loop of one write chunk: pairs [p0, p1) of the tile, values
 // from the LDS window, run offsets from dt (LDS or global, fixed per
 // instantiation so that every access has a known address space).
 template <int FMT, class D>
-__device__ __forceinline__ double cold_copy(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ wsrc, D dt,
-                                            const typename Codec<FMT>::store_t *win,
-                                            typename Codec<FMT>::store_t *__restrict__ vals) {
-    using S = typename Codec<FMT>::store_t;
-    double ms = 0.0;
+__device__ __forceinline__ void cold_copy(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ wsrc, D dt,
+                                          const typename Codec<FMT>::store_t *win,
+                                          typename Codec<FMT>::store_t *__restrict__ vals) {
     // from p0 rounded down to 64: every wave reads whole cache lines
     for (uint32_t base = p0 & ~63u; base < p1; base += CW_THREADS * CW_UNROLL) {
         uint32_t wv[CW_UNROLL];
@@ -383,14 +463,9 @@ __device__ __forceinline__ double cold_copy(uint32_t p0, uint32_t p1, const uint
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
-            if (p >= p0 && p < p1) {
-                const S v = win[wv[u] & 0x7fffu];
-                vals[p + dl[u]] = v;
-                ms += (double)fabsf(Codec<FMT>::dec(v));
-            }
+            if (p >= p0 && p < p1) vals[p + dl[u]] = win[wv[u] & 0x7fffu];
         }
     }
-    return ms;
 }
 
 template <int FMT>
@@ -398,44 +473,52 @@ __global__ void __launch_bounds__(CW_THREADS)
 k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
              uint32_t nbins, const uint32_t *__restrict__ wsrc, const uint32_t *__restrict__ delta,
              const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
-             double *__restrict__ mass, const PrScalars *__restrict__ prs) {
+             const uint32_t *__restrict__ deg, double *__restrict__ mass, const PrScalars *__restrict__ prs) {
     using S = typename Codec<FMT>::store_t;
     __shared__ S win[COLD_TILE_V];
     __shared__ uint32_t dls[CW_DELTA_LDS];
     __shared__ double red[CW_THREADS / 64];
     const uint32_t c = xcd_chunk(blockIdx.x, nwchunks);
-    const uint32_t t = wchunk[4 * c], p0 = wchunk[4 * c + 1], p1 = wchunk[4 * c + 2];
+    const uint32_t t = wchunk[4 * c], p0 = wchunk[4 * c + 1], p1 = wchunk[4 * c + 2], first = wchunk[4 * c + 3];
     // a tile's sources span < COLD_TILE_V IDs; tile_lo[t+1] may lie further
     // on when sources without edges follow the tile
     const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
-    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) win[i] = contrib[lo + i];
+    // the tile's first chunk: the mass of its edges, sum of |contribution|
+    // x out-degree over its sources (a value is added once per edge)
+    double ms = 0.0;
+    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) {
+        const S v = contrib[lo + i];
+        win[i] = v;
+        if (first) ms += (double)fabsf(Codec<FMT>::dec(v)) * (double)deg[lo + i];
+    }
     const uint32_t *dt = delta + (size_t)t * nbins;
     const bool staged = nbins <= CW_DELTA_LDS;   // the tile's run offsets fit in LDS
     if (staged)
         for (uint32_t i = threadIdx.x; i < nbins; i += CW_THREADS) dls[i] = dt[i];
     __syncthreads();
-    double ms = staged ? cold_copy<FMT>(p0, p1, wsrc, (const uint32_t *)dls, win, vals)
-                       : cold_copy<FMT>(p0, p1, wsrc, dt, win, vals);
+    if (staged) cold_copy<FMT>(p0, p1, wsrc, (const uint32_t *)dls, win, vals);
+    else cold_copy<FMT>(p0, p1, wsrc, dt, win, vals);
     ms = block_sum(ms, red);
     if (threadIdx.x == 0) mass[c] = FMT == GPR_CONTRIB_F16 ? ms / (double)prs->enc_scale : ms;
 }
 
 constexpr int CA_THREADS = 1024;
-constexpr int CA_U = 8;                          // 64-entry blocks of one wave in flight
+constexpr int CA_U = 16;                         // 64-entry blocks of one wave in flight
 
-// v * 2^-E truncated toward zero, as an integer, from the float's bits:
-// v = m * 2^(e-150) for a normal float (m the 24-bit significand), so the
-// result is m shifted by e - 150 - E = e + shift.  The unit is chosen so
-// that |v| * 2^-E < 2^61 for every contribution, so the shift stays below
-// 38; zeros, subnormals (far below the unit) and non-finite values give 0.
-__device__ __forceinline__ long long to_fixed(float v, int shift) {
-    const uint32_t bits = __float_as_uint(v);
-    const int e = (int)((bits >> 23) & 0xffu);
-    if (e == 0 || e == 0xff) return 0;
-    const uint64_t m = (uint64_t)((bits & 0x7fffffu) | 0x800000u);
-    const int s = e + shift;
-    const uint64_t mag = s >= 0 ? (s < 40 ? m << s : 0ull) : (s > -25 ? m >> (-s) : 0ull);
-    return (bits >> 31) ? -(long long)mag : (long long)mag;
+// v * 2^-E truncated toward zero, as an integer (sh = -E - 32).  The unit
+// is chosen so that |v| * 2^-E < 2^61 for every contribution, so
+// t = |v| * 2^-E / 2^32 < 2^29; scaling by a power of two, floor and the
+// subtraction of the floor are exact in FP32 (t has 24 significant bits),
+// so hi = floor(t) and lo = trunc((t - hi) * 2^32) are the two 32-bit halves
+// of the integer.  (A result below 2^-126 may lose bits to a subnormal, but
+// its integer part is 0 either way.)
+__device__ __forceinline__ long long to_fixed(float v, int sh) {
+    const float t = ldexpf(fabsf(v), sh);
+    const float h = floorf(t);
+    const uint32_t hi = (uint32_t)h;
+    const uint32_t lo = (uint32_t)ldexpf(t - h, 32);
+    const uint64_t mag = ((uint64_t)hi << 32) | lo;
+    return v < 0.0f ? -(long long)mag : (long long)mag;
 }
 
 // y[d] (+)= s * unit, and the running sum of y and maximum of y/deg.
@@ -467,13 +550,21 @@ __device__ __forceinline__ void block_emit(double my_sum, float my_max, double *
     }
 }
 
+// A stored value's bits in a 32-bit register and back (shuffles move bits).
+__device__ __forceinline__ uint32_t store_bits(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint32_t store_bits(uint16_t v) { return v; }
+template <class S> __device__ __forceinline__ S store_from_bits(uint32_t b);
+template <> __device__ __forceinline__ float store_from_bits<float>(uint32_t b) { return __uint_as_float(b); }
+template <> __device__ __forceinline__ uint16_t store_from_bits<uint16_t>(uint32_t b) { return (uint16_t)b; }
+
 template <int FMT>
 __global__ void __launch_bounds__(CA_THREADS)
-k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
-             const uint16_t *__restrict__ cdst, const typename Codec<FMT>::store_t *__restrict__ vals,
+k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cdst, const uint32_t *bbase,
+             const typename Codec<FMT>::store_t *vals,
              const uint32_t *__restrict__ deg, float *__restrict__ y, long long *__restrict__ partials, int add,
              double *__restrict__ part, float *__restrict__ part_max, const PrScalars *__restrict__ prs) {
     using C = Codec<FMT>;
+    using S = typename C::store_t;
     __shared__ unsigned long long acc[COLD_BIN];
     __shared__ double dred[CA_THREADS / 64];
     __shared__ float fred[CA_THREADS / 64];
@@ -484,59 +575,67 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n,
     const double unit = prs->cold_unit;
     const float dscale = FMT == GPR_CONTRIB_F16 ? 1.0f / prs->enc_scale : 1.0f;
     const unsigned w = threadIdx.x / 64, ln = lane(), nw = CA_THREADS / 64;
-    const int shift = -150 - prs->cold_exp;
+    const int shift = -prs->cold_exp - 32;
     // 32-bit offsets from the chunk's first entry rounded down to 64, so
-    // that every wave reads whole cache lines no other wave reads
+    // that every wave reads whole cache lines no other wave reads, and a
+    // wave's 64 entries are one block of the pair bases.  Lanes before the
+    // chunk read their own entries (their pair flags count); lanes past its
+    // end read the first entry, with the flag dropped.
+    //
+    // A block's entries use pairs pb .. pb+63 at most (pb = the pair of its
+    // first entry), so lane l loads vals[pb + l] beside its destination, and
+    // takes the value of pair pb + r, r = pair starts among lanes 1..l,
+    // from lane r: no load waits for another.  The next iteration's bases
+    // are loaded one iteration ahead.
     const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
     const uint16_t *kb = cdst + a0;
-    const typename C::store_t *vb = vals + a0;
-    if constexpr (sizeof(typename C::store_t) == 2) {
-        // 16-bit values: a lane takes two neighbouring entries per load
-        // (4 B of destinations and 4 B of values; a0 is 64-aligned, and the
-        // arrays carry a 64-entry tail so the last pair is in bounds)
-        const uint32_t *kb2 = (const uint32_t *)kb;
-        const uint32_t *vb2 = (const uint32_t *)vb;
-        const uint32_t len2 = (len + 1) / 2, lo2 = lo / 2;
-        for (uint32_t base = w * 64; base < len2; base += nw * 64 * CA_U) {
-            uint32_t kp[CA_U], vp[CA_U];
+    const uint32_t *bb = bbase + (a0 >> 6);
+    const uint32_t step = nw * 64 * CA_U;
+    uint32_t pbn[CA_U];
 #pragma unroll
-            for (int u = 0; u < CA_U; ++u) {
-                const uint32_t q = base + u * nw * 64 + ln;
-                const uint32_t qc = q >= lo2 && q < len2 ? q : lo2;
-                kp[u] = __builtin_nontemporal_load(&kb2[qc]);
-                vp[u] = __builtin_nontemporal_load(&vb2[qc]);
-            }
-#pragma unroll
-            for (int u = 0; u < CA_U; ++u) {
-                const uint32_t i = 2 * (base + u * nw * 64 + ln);
-#pragma unroll
-                for (int h = 0; h < 2; ++h) {
-                    const long long fx = to_fixed(C::dec((uint16_t)(vp[u] >> (16 * h))) * dscale, shift);
-                    if (i + h >= lo && i + h < len && fx != 0)
-                        atomicAdd(&acc[(kp[u] >> (16 * h)) & 0xffffu], (unsigned long long)fx);
-                }
-            }
-        }
-    } else
-    for (uint32_t base = w * 64; base < len; base += nw * 64 * CA_U) {
+    for (int u = 0; u < CA_U; ++u) {
+        const uint32_t ob = w * 64 + u * nw * 64;
+        pbn[u] = bb[ob < len ? ob >> 6 : 0u];
+    }
+    for (uint32_t base = w * 64; base < len; base += step) {
         uint16_t key[CA_U];
-        typename C::store_t raw[CA_U];
-        // unconditional loads (masked lanes read the chunk's first entry),
-        // so that all of them are in flight together
+        S vv[CA_U];
+        // a block wholly past the end adds nothing: base 0 keeps its value
+        // loads in bounds (the array carries a 64-value tail)
+#pragma unroll
+        for (int u = 0; u < CA_U; ++u) {
+            const uint32_t ob = base + u * nw * 64, o = ob + ln;
+            key[u] = __builtin_nontemporal_load(&kb[o < len ? o : 0u]);
+            vv[u] = vals[(ob < len ? pbn[u] : 0u) + ln];
+        }
+#pragma unroll
+        for (int u = 0; u < CA_U; ++u) {
+            const uint32_t ob = base + step + u * nw * 64;
+            pbn[u] = bb[ob < len ? ob >> 6 : 0u];
+        }
+        // keep every load above issued before the first use below (the
+        // asm stops the loads from being sunk to their uses; the three
+        // arrays are not __restrict__ for this reason)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        // every lane's value first (the shuffles are in flight together),
+        // then the atomics
+        uint32_t vb[CA_U];
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
             const uint32_t o = base + u * nw * 64 + ln;
-            const uint32_t oc = o >= lo && o < len ? o : lo;
-            key[u] = __builtin_nontemporal_load(&kb[oc]);
-            raw[u] = __builtin_nontemporal_load(&vb[oc]);
+            const bool f = o < len && (key[u] & 0x8000u);
+            const uint64_t m = __ballot(f);
+            const uint32_t r = mbcnt(m) + (f ? 1u : 0u) - (uint32_t)(m & 1ull);
+            vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)store_bits(vv[u]));
         }
         // LDS atomics are exact whatever the order and whatever the number
         // of lanes on one address (the hub destinations)
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
             const uint32_t o = base + u * nw * 64 + ln;
-            const long long fx = to_fixed(C::dec(raw[u]) * dscale, shift);
-            if (o >= lo && o < len && fx != 0) atomicAdd(&acc[key[u]], (unsigned long long)fx);
+            const long long fx = to_fixed(C::dec(store_from_bits<S>(vb[u])) * dscale, shift);
+            if (o >= lo && o < len && fx != 0) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
         }
     }
     __syncthreads();
@@ -597,8 +696,8 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
 #define CW(F)                                                                                                   \
     hipLaunchKernelGGL(k_cold_write<F>, dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,           \
                        L->nwchunks, L->tile_lo, L->nbins, L->wsrc, L->delta,                                    \
-                       (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, L->mass, \
-                       g->prs)
+                       (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, g->deg,  \
+                       L->mass, g->prs)
     switch (fmt) {
     case GPR_CONTRIB_F32: CW(GPR_CONTRIB_F32); break;
     case GPR_CONTRIB_BF16: CW(GPR_CONTRIB_BF16); break;
@@ -615,8 +714,8 @@ int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, fl
     ColdLayout *L = g->cold;
 #define CA(F)                                                                                                    \
     hipLaunchKernelGGL(k_cold_accum<F>, dim3(L->nachunks), dim3(CA_THREADS), 0, g->stream, L->achunk, g->n,       \
-                       L->cdst, (const typename Codec<F>::store_t *)L->vals, g->deg, y, L->partials, add, part,   \
-                       part_max, g->prs)
+                       L->cdst, L->bbase, (const typename Codec<F>::store_t *)L->vals, g->deg, y, L->partials,    \
+                       add, part, part_max, g->prs)
     switch (fmt) {
     case GPR_CONTRIB_F32: CA(GPR_CONTRIB_F32); break;
     case GPR_CONTRIB_BF16: CA(GPR_CONTRIB_BF16); break;
