@@ -115,6 +115,7 @@ constexpr uint32_t COLD_TILE_V = 32768;
 constexpr uint64_t COLD_TILE_E = 1ull << 24;
 constexpr uint64_t COLD_WCHUNK = 262144;   // pairs per write chunk
 constexpr uint64_t COLD_CHUNK_E = 1u << 20;
+constexpr uint64_t COLD_PAD_E = 32768;          // zero tail of the destination array (and of the pair bases / 64)
 constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
 struct ColdLayout {
     uint32_t from = 0;              // first cold source
@@ -124,8 +125,9 @@ struct ColdLayout {
     uint32_t *tile_lo = nullptr;    // ntiles+1 source boundaries
     uint32_t *delta = nullptr;      // ntiles x nbins: bin-major pair slot - tile-major pair position of cell (t,b)
     uint32_t *wsrc = nullptr;       // per pair, tile-major: bin << 15 | (source - tile_lo[t])
-    uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN, bit 15 = first entry of a pair
-    uint32_t *bbase = nullptr;      // per 64 bin-major entries: the pair of the first entry
+    uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN, bit 15 = first entry of a pair;
+                                    // COLD_PAD_E zero entries follow
+    uint32_t *bbase = nullptr;      // per 64 bin-major entries: the pair of the first entry (zero tail)
     uint32_t *vals = nullptr;       // per pair, bin-major: the staged contribution (4 or 2 bytes)
     uint32_t nwchunks = 0, nachunks = 0, nmulti = 0;
     uint32_t *wchunk = nullptr;     // per write chunk: tile, first pair, end pair (tile-major), first chunk of tile

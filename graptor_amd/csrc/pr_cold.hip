@@ -380,9 +380,15 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
             hipMemcpyAsync(edelta, edel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
                 hipSuccess) { r = GPR_E_HIP; break; }
         if ((r = dev_alloc((void **)&L->wsrc, 4 * npairs, "cold write words"))) break;
-        if ((r = dev_alloc((void **)&L->cdst, 2 * (ncold + 64), "cold destinations"))) break;
-        if ((r = dev_alloc((void **)&L->bbase, 4 * (ncold / 64 + 2), "cold pair bases"))) break;
-        if (hipMemsetAsync(L->cdst, 0, 2 * (ncold + 64), g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        // zero tails: the accumulation reads up to two loop steps past a
+        // chunk's end without bounds checks
+        if ((r = dev_alloc((void **)&L->cdst, 2 * (ncold + COLD_PAD_E), "cold destinations"))) break;
+        if ((r = dev_alloc((void **)&L->bbase, 4 * (ncold / 64 + 1 + COLD_PAD_E / 64), "cold pair bases"))) break;
+        if (hipMemsetAsync(L->cdst, 0, 2 * (ncold + COLD_PAD_E), g->stream) != hipSuccess ||
+            hipMemsetAsync(L->bbase, 0, 4 * (ncold / 64 + 1 + COLD_PAD_E / 64), g->stream) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
         hipLaunchKernelGGL(k_cold_finish, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), incl, ncold,
                            nbins, edelta, L->delta, L->wsrc, L->cdst, L->bbase);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
@@ -433,6 +439,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
 // ---- per-iteration kernels -------------------------------------------------
 constexpr int CW_THREADS = 1024;
 constexpr int CW_UNROLL = 8;
+constexpr int CW_STAGE = 8;               // window loads per thread in flight
 constexpr uint32_t CW_DELTA_LDS = 4096;   // run offsets staged in LDS up to this many bins
 
 // Workgroup i -> chunk: the XCD that runs workgroup i (i mod 8 on this
@@ -468,6 +475,36 @@ __device__ __forceinline__ void cold_copy(uint32_t p0, uint32_t p1, const uint32
     }
 }
 
+// Stage a tile's w contributions in LDS, CW_STAGE loads per thread at a
+// time (clamped, so that all are in flight together); with MASS, also this
+// thread's share of sum |contribution| x out-degree.
+template <int FMT, bool MASS>
+__device__ __forceinline__ double cold_stage(const typename Codec<FMT>::store_t *__restrict__ c,
+                                             const uint32_t *__restrict__ deg, uint32_t w,
+                                             typename Codec<FMT>::store_t *win) {
+    using S = typename Codec<FMT>::store_t;
+    double ms = 0.0;
+    for (uint32_t i0 = 0; i0 < w; i0 += CW_THREADS * CW_STAGE) {
+        S v[CW_STAGE];
+        uint32_t dg[CW_STAGE];
+#pragma unroll
+        for (int k = 0; k < CW_STAGE; ++k) {
+            const uint32_t i = i0 + k * CW_THREADS + threadIdx.x, ic = i < w ? i : 0u;
+            v[k] = c[ic];
+            if constexpr (MASS) dg[k] = deg[ic];
+        }
+#pragma unroll
+        for (int k = 0; k < CW_STAGE; ++k) {
+            const uint32_t i = i0 + k * CW_THREADS + threadIdx.x;
+            if (i < w) {
+                win[i] = v[k];
+                if constexpr (MASS) ms += (double)fabsf(Codec<FMT>::dec(v[k])) * (double)dg[k];
+            }
+        }
+    }
+    return ms;
+}
+
 template <int FMT>
 __global__ void __launch_bounds__(CW_THREADS)
 k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
@@ -485,12 +522,8 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
     // the tile's first chunk: the mass of its edges, sum of |contribution|
     // x out-degree over its sources (a value is added once per edge)
-    double ms = 0.0;
-    for (uint32_t i = threadIdx.x; i < w; i += CW_THREADS) {
-        const S v = contrib[lo + i];
-        win[i] = v;
-        if (first) ms += (double)fabsf(Codec<FMT>::dec(v)) * (double)deg[lo + i];
-    }
+    const double ms0 = first ? cold_stage<FMT, true>(contrib + lo, deg + lo, w, win)
+                             : cold_stage<FMT, false>(contrib + lo, deg + lo, w, win);
     const uint32_t *dt = delta + (size_t)t * nbins;
     const bool staged = nbins <= CW_DELTA_LDS;   // the tile's run offsets fit in LDS
     if (staged)
@@ -498,12 +531,13 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     __syncthreads();
     if (staged) cold_copy<FMT>(p0, p1, wsrc, (const uint32_t *)dls, win, vals);
     else cold_copy<FMT>(p0, p1, wsrc, dt, win, vals);
-    ms = block_sum(ms, red);
+    const double ms = block_sum(ms0, red);
     if (threadIdx.x == 0) mass[c] = FMT == GPR_CONTRIB_F16 ? ms / (double)prs->enc_scale : ms;
 }
 
 constexpr int CA_THREADS = 1024;
 constexpr int CA_U = 16;                         // 64-entry blocks of one wave in flight
+constexpr uint32_t nw_max = CA_THREADS / 64;
 
 // v * 2^-E truncated toward zero, as an integer (sh = -E - 32).  The unit
 // is chosen so that |v| * 2^-E < 2^61 for every contribution, so
@@ -550,6 +584,21 @@ __device__ __forceinline__ void block_emit(double my_sum, float my_max, double *
     }
 }
 
+// Raw buffer descriptor over `bytes` bytes from p (gfx9 word 3: 32-bit
+// untyped data); a load past the range returns 0.
+constexpr int BUF_WORD3 = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes, BUF_WORD3);
+}
+constexpr int BUF_NT = 2;   // cache policy of a streamed read (slc: non-temporal)
+
+__device__ __forceinline__ uint32_t buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, float) {
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ uint32_t buf_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, uint16_t) {
+    return (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+}
+
 // A stored value's bits in a 32-bit register and back (shuffles move bits).
 __device__ __forceinline__ uint32_t store_bits(float v) { return __float_as_uint(v); }
 __device__ __forceinline__ uint32_t store_bits(uint16_t v) { return v; }
@@ -574,45 +623,48 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     __syncthreads();
     const double unit = prs->cold_unit;
     const float dscale = FMT == GPR_CONTRIB_F16 ? 1.0f / prs->enc_scale : 1.0f;
-    const unsigned w = threadIdx.x / 64, ln = lane(), nw = CA_THREADS / 64;
+    // the wave index through readfirstlane: known uniform, so the offsets
+    // and pair bases derived from it live in scalar registers
+    const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x / 64), ln = lane(), nw = CA_THREADS / 64;
     const int shift = -prs->cold_exp - 32;
     // 32-bit offsets from the chunk's first entry rounded down to 64, so
     // that every wave reads whole cache lines no other wave reads, and a
-    // wave's 64 entries are one block of the pair bases.  Lanes before the
-    // chunk read their own entries (their pair flags count); lanes past its
-    // end read the first entry, with the flag dropped.
+    // wave's 64 entries are one block of the pair bases.  Reads run past the
+    // chunk (into the next one, or the arrays' zero tails) without bounds
+    // checks; only the atomics are limited to the chunk.
     //
     // A block's entries use pairs pb .. pb+63 at most (pb = the pair of its
-    // first entry), so lane l loads vals[pb + l] beside its destination, and
-    // takes the value of pair pb + r, r = pair starts among lanes 1..l,
-    // from lane r: no load waits for another.  The next iteration's bases
-    // are loaded one iteration ahead.
+    // first entry), so lane l loads vals[pb + l] beside its destination and
+    // takes the value of pair pb + r from lane r, r = the pair starts among
+    // lanes 1..l (entries past the chunk sit in higher lanes and never
+    // count): no load waits for another.  The next iteration's bases are
+    // loaded one iteration ahead.
     const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
-    const uint16_t *kb = cdst + a0;
     const uint32_t *bb = bbase + (a0 >> 6);
     const uint32_t step = nw * 64 * CA_U;
+    static_assert(2 * nw_max * 64 * CA_U <= COLD_PAD_E, "zero tail shorter than two loop steps");
+    // buffer descriptors from the chunk's first entry and first pair: the
+    // uniform part of each offset goes into the scalar offset, and reads
+    // past the end (zero-tail blocks use pair base 0) return 0
+    const __amdgpu_buffer_rsrc_t krs = buf_rsrc(cdst + a0, 2u * (len + 2 * step));
+    const uint32_t pfirst = __builtin_amdgcn_readfirstlane(bb[0]);
+    const __amdgpu_buffer_rsrc_t vrs = buf_rsrc(vals + pfirst, 0x7fffffffu);
     uint32_t pbn[CA_U];
 #pragma unroll
-    for (int u = 0; u < CA_U; ++u) {
-        const uint32_t ob = w * 64 + u * nw * 64;
-        pbn[u] = bb[ob < len ? ob >> 6 : 0u];
-    }
+    for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(w * 64 + u * nw * 64) >> 6];
     for (uint32_t base = w * 64; base < len; base += step) {
-        uint16_t key[CA_U];
-        S vv[CA_U];
-        // a block wholly past the end adds nothing: base 0 keeps its value
-        // loads in bounds (the array carries a 64-value tail)
+        uint32_t key[CA_U];
+        uint32_t vv[CA_U];
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            const uint32_t ob = base + u * nw * 64, o = ob + ln;
-            key[u] = __builtin_nontemporal_load(&kb[o < len ? o : 0u]);
-            vv[u] = vals[(ob < len ? pbn[u] : 0u) + ln];
+            key[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(krs, (int)(2 * ln), (int)(2 * (base + u * nw * 64)),
+                                                                    BUF_NT);
+            // the base was loaded an iteration ago; readfirstlane makes it
+            // scalar (all lanes hold the same value)
+            vv[u] = buf_load(vrs, sizeof(S) * ln, sizeof(S) * (__builtin_amdgcn_readfirstlane(pbn[u]) - pfirst), S());
         }
 #pragma unroll
-        for (int u = 0; u < CA_U; ++u) {
-            const uint32_t ob = base + step + u * nw * 64;
-            pbn[u] = bb[ob < len ? ob >> 6 : 0u];
-        }
+        for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(base + step + u * nw * 64) >> 6];
         // keep every load above issued before the first use below (the
         // asm stops the loads from being sunk to their uses; the three
         // arrays are not __restrict__ for this reason)
@@ -623,11 +675,9 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
         uint32_t vb[CA_U];
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            const uint32_t o = base + u * nw * 64 + ln;
-            const bool f = o < len && (key[u] & 0x8000u);
-            const uint64_t m = __ballot(f);
-            const uint32_t r = mbcnt(m) + (f ? 1u : 0u) - (uint32_t)(m & 1ull);
-            vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)store_bits(vv[u]));
+            const uint64_t m = __ballot(key[u] & 0x8000u);
+            const uint32_t r = mbcnt(m >> 1);
+            vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)vv[u]);
         }
         // LDS atomics are exact whatever the order and whatever the number
         // of lanes on one address (the hub destinations)
@@ -635,7 +685,7 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
         for (int u = 0; u < CA_U; ++u) {
             const uint32_t o = base + u * nw * 64 + ln;
             const long long fx = to_fixed(C::dec(store_from_bits<S>(vb[u])) * dscale, shift);
-            if (o >= lo && o < len && fx != 0) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
+            if (o >= lo && o < len) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
         }
     }
     __syncthreads();
