@@ -81,14 +81,12 @@ __device__ __forceinline__ uint32_t cold_row_of(const uint64_t *index, uint32_t 
 
 // Over the cold edges in tiles of CK_EDGES consecutive edges (balanced
 // whatever the degrees; the rows of a tile found through LDS-staged offsets
-// when they fit).  mode 0: count entries per (tile, bin).  mode 1: write
-// the sort key tile*nbins + bin and the word (source offset in the tile) << 14
-// | (destination in bin) at the edge's position.
+// when they fit): the sort key tile*nbins + bin and the word (source offset
+// in the tile) << 14 | (destination in bin) at the edge's position.
 constexpr int CK_EDGES = 4096;
 __global__ void __launch_bounds__(256)
-k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
-            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *cnt, uint32_t *keys,
-            uint32_t *vals) {
+k_cold_keys(const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
+            const uint32_t *__restrict__ tile_lo, uint32_t ntiles, uint32_t nbins, uint32_t *keys, uint32_t *vals) {
     __shared__ uint64_t sidx[CK_EDGES + 1];
     __shared__ uint32_t s_r0, s_cnt, s_t0;
     const uint32_t first = tile_lo[0], last = tile_lo[ntiles];
@@ -126,27 +124,35 @@ k_cold_keys(int mode, const uint64_t *__restrict__ index, const uint32_t *__rest
             const uint32_t v = nbrs[e];
             const uint32_t b = v / COLD_BIN;
             const uint32_t cell = t * nbins + b;   // < 2^32: checked by the host
-            if (mode == 0) {
-                atomicAdd(&cnt[cell], 1u);
-            } else {
-                keys[e - E0] = cell;
-                vals[e - E0] = ((u - tile_lo[t]) << 14) | (v - b * COLD_BIN);
-            }
+            keys[e - E0] = cell;
+            vals[e - E0] = ((u - tile_lo[t]) << 14) | (v - b * COLD_BIN);
         }
         __syncthreads();
     }
 }
 
 // Sorted entry p (tile-major, a source's edges of one bin together): 1 when
-// it starts a pair (the first entry of its cell, or a new source), counted
-// per cell.
+// it starts a pair (the first entry of its cell, or a new source); the
+// first entry of a cell also records its position (no atomics: each cell
+// has one first entry).
 __global__ void k_cold_flags(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ words, uint64_t count,
-                             uint32_t *flag, uint32_t *pcnt) {
+                             uint32_t *flag, uint32_t *cstart) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < count; p += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t k = keys[p];
-        const bool f = p == 0 || k != keys[p - 1] || (words[p] >> 14) != (words[p - 1] >> 14);
+        const bool first = p == 0 || k != keys[p - 1];
+        const bool f = first || (words[p] >> 14) != (words[p - 1] >> 14);
         flag[p] = f ? 1u : 0u;
-        if (f) atomicAdd(&pcnt[k], 1u);
+        if (first) cstart[k] = (uint32_t)p;
+    }
+}
+
+// Per cell with entries: the pairs before it (its first entry starts a
+// pair, so incl there counts it too).
+__global__ void k_cold_cellpairs(const uint32_t *__restrict__ cstart, uint64_t cells,
+                                 const uint32_t *__restrict__ incl, uint32_t *pstart) {
+    for (uint64_t c = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; c < cells; c += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = cstart[c];
+        pstart[c] = s == 0xffffffffu ? 0xffffffffu : incl[s] - 1u;
     }
 }
 
@@ -279,17 +285,79 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     const unsigned eg = grid_for(ncold, 256, 65536);
     do {
         if ((r = dev_alloc((void **)&L->tile_lo, sizeof(uint32_t) * lo.size(), "cold tiles"))) break;
-        if ((r = dev_alloc((void **)&cnt, sizeof(uint32_t) * cells, "cold counts"))) break;
         if (hipMemcpyAsync(L->tile_lo, lo.data(), sizeof(uint32_t) * lo.size(), hipMemcpyHostToDevice, g->stream) !=
             hipSuccess) { r = GPR_E_HIP; break; }
-        // ---- edges per cell: the bin-major entry positions
-        if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 0, g->index, g->nbrs, L->tile_lo, ntiles,
-                           nbins, cnt, (uint32_t *)nullptr, (uint32_t *)nullptr);
-        ecnt.resize(cells);
-        if (hipMemcpyAsync(ecnt.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) !=
+        // ---- edges sorted by (tile, bin), stable: a source's edges of one
+        // bin stay together
+        if ((r = dev_alloc((void **)&k0, 4 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&k1, 4 * ncold, "cold sort keys"))) break;
+        if ((r = dev_alloc((void **)&v0, 4 * ncold, "cold sort words"))) break;
+        if ((r = dev_alloc((void **)&v1, 4 * ncold, "cold sort words"))) break;
+        hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, g->index, g->nbrs, L->tile_lo, ntiles,
+                           nbins, k0, v0);
+        rocprim::double_buffer<uint32_t> kb(k0, k1);
+        rocprim::double_buffer<uint32_t> vb(v0, v1);
+        const unsigned end_bit = bits_for(cells);
+        size_t tsz = 0;
+        if (rocprim::radix_sort_pairs(nullptr, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
+        if ((r = dev_alloc(&tmp, tsz, "cold sort workspace"))) break;
+        if (rocprim::radix_sort_pairs(tmp, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
+            r = GPR_E_HIP;
+            break;
+        }
+        dev_free(tmp);
+        tmp = nullptr;
+        // ---- pair flags (into the sort's spare key buffer), cell starts,
+        // the flags' prefix sum (into the spare word buffer), pairs before
+        // each cell
+        uint32_t *flag = kb.alternate(), *incl = vb.alternate();
+        if ((r = dev_alloc((void **)&cnt, sizeof(uint32_t) * 2 * cells, "cold cell starts"))) break;
+        uint32_t *cstart = cnt, *pstart = cnt + cells;
+        if (hipMemsetAsync(cstart, 0xff, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        hipLaunchKernelGGL(k_cold_flags, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), ncold, flag,
+                           cstart);
+        tsz = 0;
+        if (rocprim::inclusive_scan(nullptr, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        if ((r = dev_alloc(&tmp, tsz, "cold scan workspace"))) break;
+        if (rocprim::inclusive_scan(tmp, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
+            hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(tmp);
+        tmp = nullptr;
+        hipLaunchKernelGGL(k_cold_cellpairs, dim3(grid_for(cells, 256, 65536)), dim3(256), 0, g->stream, cstart,
+                           (uint64_t)cells, incl, pstart);
+        std::vector<uint32_t> hstart(2 * cells);
+        uint32_t ptotal = 0;
+        if (hipMemcpyAsync(hstart.data(), cnt, sizeof(uint32_t) * 2 * cells, hipMemcpyDeviceToHost, g->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(&ptotal, incl + (ncold - 1), sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream) !=
                 hipSuccess ||
             hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
+        dev_free(cnt);
+        cnt = nullptr;
+        // ---- counts per cell from the starts of consecutive non-empty cells
+        // (cells are numbered in the sorted order)
+        ecnt.assign(cells, 0);
+        pcnt.assign(cells, 0);
+        {
+            size_t prev = cells;
+            for (size_t c = 0; c <= cells; ++c) {
+                if (c < cells && hstart[c] == 0xffffffffu) continue;
+                const uint64_t es = c < cells ? hstart[c] : ncold;
+                const uint64_t ps = c < cells ? hstart[cells + c] : ptotal;
+                if (prev < cells) {
+                    ecnt[prev] = (uint32_t)(es - hstart[prev]);
+                    pcnt[prev] = (uint32_t)(ps - hstart[cells + prev]);
+                }
+                prev = c;
+            }
+        }
+        hstart.clear();
+        hstart.shrink_to_fit();
+        // ---- entry plan: bin-major entry positions, accumulation chunks
         std::vector<uint64_t> ebstart;
         plan_runs(ecnt, ntiles, nbins, edel, ebstart);
         ecnt.clear();
@@ -311,49 +379,6 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
                 nslots += k;
             }
         }
-        // ---- edges sorted by (tile, bin), stable: a source's edges of one
-        // bin stay together
-        if ((r = dev_alloc((void **)&k0, 4 * ncold, "cold sort keys"))) break;
-        if ((r = dev_alloc((void **)&k1, 4 * ncold, "cold sort keys"))) break;
-        if ((r = dev_alloc((void **)&v0, 4 * ncold, "cold sort words"))) break;
-        if ((r = dev_alloc((void **)&v1, 4 * ncold, "cold sort words"))) break;
-        hipLaunchKernelGGL(k_cold_keys, dim3(pg), dim3(256), 0, g->stream, 1, g->index, g->nbrs, L->tile_lo, ntiles,
-                           nbins, (uint32_t *)nullptr, k0, v0);
-        rocprim::double_buffer<uint32_t> kb(k0, k1);
-        rocprim::double_buffer<uint32_t> vb(v0, v1);
-        const unsigned end_bit = bits_for(cells);
-        size_t tsz = 0;
-        if (rocprim::radix_sort_pairs(nullptr, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
-            r = GPR_E_HIP;
-            break;
-        }
-        if ((r = dev_alloc(&tmp, tsz, "cold sort workspace"))) break;
-        if (rocprim::radix_sort_pairs(tmp, tsz, kb, vb, (size_t)ncold, 0, end_bit, g->stream) != hipSuccess) {
-            r = GPR_E_HIP;
-            break;
-        }
-        dev_free(tmp);
-        tmp = nullptr;
-        // ---- pair flags (into the sort's spare key buffer), pairs per cell,
-        // and the flags' prefix sum (into the spare word buffer)
-        uint32_t *flag = kb.alternate(), *incl = vb.alternate();
-        if (hipMemsetAsync(cnt, 0, sizeof(uint32_t) * cells, g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        hipLaunchKernelGGL(k_cold_flags, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), ncold, flag,
-                           cnt);
-        tsz = 0;
-        if (rocprim::inclusive_scan(nullptr, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
-            hipSuccess) { r = GPR_E_HIP; break; }
-        if ((r = dev_alloc(&tmp, tsz, "cold scan workspace"))) break;
-        if (rocprim::inclusive_scan(tmp, tsz, flag, incl, (size_t)ncold, rocprim::plus<uint32_t>(), g->stream) !=
-            hipSuccess) { r = GPR_E_HIP; break; }
-        dev_free(tmp);
-        tmp = nullptr;
-        pcnt.resize(cells);
-        if (hipMemcpyAsync(pcnt.data(), cnt, sizeof(uint32_t) * cells, hipMemcpyDeviceToHost, g->stream) !=
-                hipSuccess ||
-            hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
-        dev_free(cnt);
-        cnt = nullptr;
         // ---- pair plan: bin-major pair slots, write chunks per tile
         {
             std::vector<uint64_t> pbstart;
@@ -609,7 +634,7 @@ template <> __device__ __forceinline__ uint16_t store_from_bits<uint16_t>(uint32
 template <int FMT>
 __global__ void __launch_bounds__(CA_THREADS)
 k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cdst, const uint32_t *bbase,
-             const typename Codec<FMT>::store_t *vals,
+             const typename Codec<FMT>::store_t *vals, uint64_t npairs,
              const uint32_t *__restrict__ deg, float *__restrict__ y, long long *__restrict__ partials, int add,
              double *__restrict__ part, float *__restrict__ part_max, const PrScalars *__restrict__ prs) {
     using C = Codec<FMT>;
@@ -644,11 +669,14 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     const uint32_t step = nw * 64 * CA_U;
     static_assert(2 * nw_max * 64 * CA_U <= COLD_PAD_E, "zero tail shorter than two loop steps");
     // buffer descriptors from the chunk's first entry and first pair: the
-    // uniform part of each offset goes into the scalar offset, and reads
-    // past the end (zero-tail blocks use pair base 0) return 0
+    // uniform part of each offset goes into the scalar offset.  A block past
+    // the end of the pair-base array reads base 0 (its zero tail); bases are
+    // clamped to the chunk's first pair, so no offset is negative, and the
+    // value descriptor covers exactly the pairs from there on (+ the tail)
     const __amdgpu_buffer_rsrc_t krs = buf_rsrc(cdst + a0, 2u * (len + 2 * step));
     const uint32_t pfirst = __builtin_amdgcn_readfirstlane(bb[0]);
-    const __amdgpu_buffer_rsrc_t vrs = buf_rsrc(vals + pfirst, 0x7fffffffu);
+    const uint64_t vbytes = (npairs + 64 - pfirst) * (uint64_t)sizeof(S);
+    const __amdgpu_buffer_rsrc_t vrs = buf_rsrc(vals + pfirst, vbytes < 0xffffffffull ? (uint32_t)vbytes : 0xffffffffu);
     uint32_t pbn[CA_U];
 #pragma unroll
     for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(w * 64 + u * nw * 64) >> 6];
@@ -661,7 +689,8 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
                                                                     BUF_NT);
             // the base was loaded an iteration ago; readfirstlane makes it
             // scalar (all lanes hold the same value)
-            vv[u] = buf_load(vrs, sizeof(S) * ln, sizeof(S) * (__builtin_amdgcn_readfirstlane(pbn[u]) - pfirst), S());
+            const uint32_t pb = max(__builtin_amdgcn_readfirstlane(pbn[u]), pfirst);
+            vv[u] = buf_load(vrs, sizeof(S) * ln, sizeof(S) * (pb - pfirst), S());
         }
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(base + step + u * nw * 64) >> 6];
@@ -764,8 +793,8 @@ int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, fl
     ColdLayout *L = g->cold;
 #define CA(F)                                                                                                    \
     hipLaunchKernelGGL(k_cold_accum<F>, dim3(L->nachunks), dim3(CA_THREADS), 0, g->stream, L->achunk, g->n,       \
-                       L->cdst, L->bbase, (const typename Codec<F>::store_t *)L->vals, g->deg, y, L->partials,    \
-                       add, part, part_max, g->prs)
+                       L->cdst, L->bbase, (const typename Codec<F>::store_t *)L->vals, L->npairs, g->deg, y,      \
+                       L->partials, add, part, part_max, g->prs)
     switch (fmt) {
     case GPR_CONTRIB_F32: CA(GPR_CONTRIB_F32); break;
     case GPR_CONTRIB_BF16: CA(GPR_CONTRIB_BF16); break;
