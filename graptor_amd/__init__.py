@@ -17,7 +17,8 @@ LIB_PATH = os.path.join(_HERE, "libgpr.so")
 GPR_OK, GPR_E_INVAL, GPR_E_FORMAT, GPR_E_OOM, GPR_E_HIP, GPR_E_IO, GPR_E_NODEV = range(7)
 U32, F32, F64, U64, U16 = range(5)
 FRONTIER_EMPTY, FRONTIER_ALL, FRONTIER_LIST, FRONTIER_BITMAP = range(4)
-RELAX_MIN_PLUS1, RELAX_MIN, RELAX_PR_SUM = range(3)
+RELAX_MIN_PLUS1, RELAX_MIN, RELAX_PR_SUM, RELAX_SETIF = range(4)
+UNSET = 0xFFFFFFFF   # an unset BFSv parent
 DST_NONE, DST_GT, DST_NE = range(3)
 RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE = range(3)
 DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
@@ -109,6 +110,7 @@ SIGNATURES = {
     "gpr_pr_scalars": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
                                  C.POINTER(C.c_float)]),
     "gpr_bfslvl": (C.c_int, [_P, C.c_uint32, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_bfsv": (C.c_int, [_P, C.c_uint32, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_ccv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_prv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_pr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
@@ -289,6 +291,15 @@ class Graph:
         o = opts(**kw)
         info = RunInfo()
         _check(lib().gpr_bfslvl(self.h, start, C.byref(o), _ptr(out), C.byref(info)))
+        return out, info
+
+    def bfsv(self, start, **kw):
+        """BFS parents (original IDs): the start its own parent, UNSET where
+        unreached."""
+        out = np.empty(self.n, np.uint32)
+        o = opts(**kw)
+        info = RunInfo()
+        _check(lib().gpr_bfsv(self.h, start, C.byref(o), _ptr(out), C.byref(info)))
         return out, info
 
     def ccv(self, **kw):

@@ -2,6 +2,7 @@
 // the reference benchmarks are written against Graptor's api:: templates.
 //
 //   BFSLVLv   bench/BFSLVLv.C:110-254  (run loop :154-251)
+//   BFSv      bench/BFSv.C:88-203      (run loop :118-199)
 //   CCv       bench/CCv.C:221-588      (run :221-471, emap_step :520-588)
 //   PRv       bench/PRv.C:115-290      (iteration :201-280)
 //   PR_vary   bench/PR_vary.C:247-757  (run_prec :247-447, staged run :652-757)
@@ -23,6 +24,7 @@ struct gpr_graph;
 namespace gpr {
 std::vector<gpr_iter_stats> &trace_of(gpr_graph *g);
 uint32_t graph_max_degree(const gpr_graph *g);
+int ids_download(gpr_graph *g, const gpr_prop *p, uint32_t *host);
 void trace_clear(gpr_graph *g);
 double stream_time_ms(gpr_graph *g, bool start);
 }
@@ -119,6 +121,49 @@ struct BFSLVLv {
             reached.del();
             all.del();
         }
+    }
+};
+
+// ---- BFSv ----------------------------------------------------------------------
+// BFS parents (BFSv.C:88-203, CONVERGENCE=1, DEFERRED_UPDATE=0): parent = ~0,
+// parent[start] = start; each step sets the unset parents of the frontier's
+// out-neighbours (api::relax(parent[d].setif(s)), the dst filter
+// msbset(parent[d]) implied by the relax) and records them.
+struct BFSv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<uint32_t> parent;
+    int iter = 0;
+    uint64_t active = 0;
+    BFSv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), parent(GA, "parent") {}
+    ~BFSv() { parent.del(); }
+
+    void run(uint32_t start, gpr_run_info *info) {
+        A::make_lazy_executor(GA).vertex_map_fill(parent, 0xffffffffu).materialize();
+        uint32_t si = 0;   // the start's own (internal) ID is its parent
+        check(gpr_graph_remap_id(GA.get(), start, &si), "remap id");
+        A::frontier F = A::frontier::create(GA, start);
+        A::make_lazy_executor(GA).vertex_map_fill(F, parent, si).materialize();
+        check(gpr_policy_reset(GA.get()), "policy reset");
+        info->edges += F.nActiveEdges();
+        iter = 0;
+        active = 1;
+        while (!F.isEmpty()) {
+            A::frontier output;
+            auto lazy = A::edgemap(GA,
+                                   A::record(output, A::reduction, A::strong),
+                                   A::filter(A::strong, A::src, F),
+                                   A::relax(A::setif(parent)),
+                                   A::config(o.direction, o.threshold_pct, o.out_kind));
+            lazy.materialize();
+            record_step(GA.get(), o, lazy.stats(), info);
+            info->edges += output.nActiveEdges();
+            active += output.nActiveVertices();
+            F.del();
+            F = output;
+            ++iter;
+        }
+        F.del();
     }
 };
 
@@ -311,6 +356,25 @@ extern "C" int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *opts
         app.run(start, &local);
         local.ms = stream_time_ms(g, false);
         if (levels) app.a_level.download(levels);
+        if (info) *info = local;
+    });
+}
+
+extern "C" int gpr_bfsv(gpr_graph *g, uint32_t start, const gpr_run_opts *opts, uint32_t *parents,
+                        gpr_run_info *info) {
+    if (!g) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    uint32_t n = 0;
+    gpr_graph_info(g, &n, nullptr, nullptr);
+    if (start >= n) return guarded([&] { check(GPR_E_INVAL, "bfsv: start vertex out of range"); });
+    return guarded([&] {
+        BFSv app(g, o);
+        gpr_run_info local = {};
+        if (o.trace) trace_clear(g);
+        stream_time_ms(g, true);
+        app.run(start, &local);
+        local.ms = stream_time_ms(g, false);
+        if (parents) check(ids_download(g, app.parent.get(), parents), "parents download");
         if (info) *info = local;
     });
 }

@@ -105,6 +105,10 @@ template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) 
     return RELAX == GPR_RELAX_MIN_PLUS1 ? ps + 1u : ps;
 }
 
+// setif (BFSv.C:146): a destination takes a value only while its top bit
+// is set (unset: ~0); the value is the source's (internal) ID
+constexpr uint32_t SETIF_OPEN = 0x80000000u;
+
 // Fused policy inference: the last workgroup to finish (by ticket) turns
 // the completed frontier counts into the next step's choice.
 __device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounters *ctr) {
@@ -191,7 +195,15 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
                 }
                 d = nbrs[index[s] + (e - pj)];
                 const uint32_t pd = prop[d];
-                if (dst_ok(pd, dkind, darg)) {
+                if constexpr (RELAX == GPR_RELAX_SETIF) {
+                    // single trigger: the CAS that replaces the open value
+                    // wins, so each destination is set and activated once
+                    if ((pd & SETIF_OPEN) && dst_ok(pd, dkind, darg) && atomicCAS(&prop[d], pd, s) == pd) {
+                        const uint64_t bit = 1ull << (d & 63);
+                        const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
+                        act = !(ow & bit);
+                    }
+                } else if (dst_ok(pd, dkind, darg)) {
                     const uint32_t nv = relax_val<RELAX>(prop[s]);
                     if (nv < pd) {
                         const uint32_t old = atomicMin(&prop[d], nv);
@@ -255,7 +267,7 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         const bool valid = d < n;
         const uint32_t orig = valid ? prop[d] : 0u;
         uint32_t cur = orig;
-        const bool act = valid && dst_ok(cur, dkind, darg);
+        const bool act = valid && dst_ok(cur, dkind, darg) && (RELAX != GPR_RELAX_SETIF || (cur & SETIF_OPEN));
         uint64_t beg = 0, end = 0;
         if (act) {
             beg = cindex[d];
@@ -278,10 +290,15 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
 #pragma unroll
                 for (int u = 0; u < PULL_UNROLL; ++u) {
                     if (live && hit[u]) {
-                        const uint32_t nv = relax_val<RELAX>(prop[s[u]]);
-                        if (nv < cur) {
-                            cur = nv;
-                            if (!dst_ok(cur, dkind, darg)) live = false;
+                        if constexpr (RELAX == GPR_RELAX_SETIF) {
+                            cur = s[u];   // the first frontier in-neighbour
+                            live = false;
+                        } else {
+                            const uint32_t nv = relax_val<RELAX>(prop[s[u]]);
+                            if (nv < cur) {
+                                cur = nv;
+                                if (!dst_ok(cur, dkind, darg)) live = false;
+                            }
                         }
                     }
                     if (live && s[u] != 0xffffffffu) ++my_scan;
@@ -307,14 +324,16 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
                     if (s[u] != 0xffffffffu) {
                         ++my_scan;
                         if (!in_bits || bit_test(in_bits, s[u])) {
-                            const uint32_t nv = relax_val<RELAX>(prop[s[u]]);
+                            // setif: the smallest frontier ID of the first
+                            // step with one (the open value is ~0)
+                            const uint32_t nv = RELAX == GPR_RELAX_SETIF ? s[u] : relax_val<RELAX>(prop[s[u]]);
                             v = nv < v ? nv : v;
                         }
                     }
                 }
                 v = wave_min(v);
                 best = v < best ? v : best;
-                if (!dst_ok(best, dkind, darg)) break;
+                if (RELAX == GPR_RELAX_SETIF ? !(best & SETIF_OPEN) : !dst_ok(best, dkind, darg)) break;
             }
             if ((int)ln == L) cur = best;
         }
@@ -487,7 +506,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     // api::fusion: a sparse step that processes what it activates at once
     // (fusion_select(true), or default_fusion_select: >= 4096 edges,
     // api.h:227-248); the record and dst filter are implied by the relax
-    const bool fuse = dir == GPR_DIR_PUSH && in &&
+    const bool fuse = dir == GPR_DIR_PUSH && in && desc->relax != GPR_RELAX_SETIF &&
                       (desc->fusion == GPR_FUSION_ON || (desc->fusion == GPR_FUSION_DEFAULT && nacte >= 4096));
     if (fuse) {
         GPR_HIP(hipEventRecord(g->ev0, g->stream));
@@ -537,14 +556,13 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         uint32_t *olist = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
         if (E) {
             const unsigned grid = grid_for(E, PUSH_TILE, (unsigned)std::max(1.0, tun().push_blocks_per_cu * num_cus()));
-            if (desc->relax == GPR_RELAX_MIN_PLUS1)
-                hipLaunchKernelGGL(k_push<GPR_RELAX_MIN_PLUS1>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
-                                   g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
-                                   desc->dst_arg, bits, olist, g->ctr, pol);
-            else
-                hipLaunchKernelGGL(k_push<GPR_RELAX_MIN>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k,
-                                   g->scan, g->scan_blk, E, g->index, g->nbrs, g->deg, prop, desc->dst_filter,
-                                   desc->dst_arg, bits, olist, g->ctr, pol);
+#define PUSH(R)                                                                                                \
+    hipLaunchKernelGGL(k_push<R>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k, g->scan, g->scan_blk, E, \
+                       g->index, g->nbrs, g->deg, prop, desc->dst_filter, desc->dst_arg, bits, olist, g->ctr, pol)
+            if (desc->relax == GPR_RELAX_MIN_PLUS1) PUSH(GPR_RELAX_MIN_PLUS1);
+            else if (desc->relax == GPR_RELAX_MIN) PUSH(GPR_RELAX_MIN);
+            else PUSH(GPR_RELAX_SETIF);
+#undef PUSH
         }
         if (olist) GPR_TRY(clear_dedup_of_list(g, olist, E < n ? E : n));
         s.edges_scanned = E;
@@ -553,14 +571,14 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         uint32_t *ol = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
         int record = out ? desc->record : GPR_RECORD_NONE;
         ProfScope ps(g, "pull");
-        if (desc->relax == GPR_RELAX_MIN_PLUS1)
-            hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN_PLUS1>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n,
-                               g->cindex, g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits,
-                               ob, ol, g->ctr, pol, (uint32_t)tun().pull_serial_max);
-        else
-            hipLaunchKernelGGL(k_pull<GPR_RELAX_MIN>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex,
-                               g->cnbrs, g->deg, prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol,
-                               g->ctr, pol, (uint32_t)tun().pull_serial_max);
+#define PULL(R)                                                                                                 \
+    hipLaunchKernelGGL(k_pull<R>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex, g->cnbrs, g->deg, \
+                       prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol, g->ctr, pol,            \
+                       (uint32_t)tun().pull_serial_max)
+        if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1);
+        else if (desc->relax == GPR_RELAX_MIN) PULL(GPR_RELAX_MIN);
+        else PULL(GPR_RELAX_SETIF);
+#undef PULL
     }
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipEventRecord(g->ev1, g->stream));
@@ -596,6 +614,7 @@ extern "C" int gpr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier
     switch (desc->relax) {
     case GPR_RELAX_MIN_PLUS1:
     case GPR_RELAX_MIN:
+    case GPR_RELAX_SETIF:
         return edgemap_minrelax(g, desc, in, out, st);
     case GPR_RELAX_PR_SUM:
         return pr_edgemap(g, desc, in, st);

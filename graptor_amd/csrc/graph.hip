@@ -292,6 +292,15 @@ __global__ void k_recode_out(const uint32_t *__restrict__ label, const uint32_t 
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         out[order ? order[i] : i] = minorig[label[i]];
 }
+// Values that are vertex IDs (BFSv parents), in original order and as
+// original IDs; values with the top bit set (unset entries) are kept.
+__global__ void k_ids_out(const uint32_t *__restrict__ v, const uint32_t *__restrict__ order, uint32_t n,
+                          uint32_t *out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t x = v[i];
+        out[order ? order[i] : i] = (x & 0x80000000u) ? x : (order ? order[x] : x);
+    }
+}
 __global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -487,6 +496,20 @@ extern "C" int gpr_prop_download(const gpr_prop *p, void *host) {
     GPR_HIP(hipStreamSynchronize(g->stream));
     return GPR_OK;
 }
+
+namespace gpr {
+int ids_download(gpr_graph *g, const gpr_prop *p, uint32_t *host) {
+    GPR_REQUIRE(p && host && p->dtype == GPR_U32 && p->g == g, "ids_download: a u32 property of this graph");
+    const uint32_t n = g->n;
+    uint32_t *out = (uint32_t *)g->scan;   // 8(n+1) bytes of scratch
+    hipLaunchKernelGGL(k_ids_out, dim3(grid_for(n, 256, 8192)), dim3(256), 0, g->stream, (const uint32_t *)p->d,
+                       g->order, n, out);
+    GPR_HIP(hipGetLastError());
+    GPR_HIP(hipMemcpyAsync(host, out, 4ull * n, hipMemcpyDeviceToHost, g->stream));
+    GPR_HIP(hipStreamSynchronize(g->stream));
+    return GPR_OK;
+}
+} // namespace gpr
 
 // CCv recode (bench/CCv.C:121-169): every vertex gets the minimum original
 // vertex ID of its component; labels[] is indexed by original ID.

@@ -4,7 +4,7 @@
 // driver.C:245-255), then post-process and validate each round and report
 // the average.  One binary serves the four benchmarks through -app.
 //
-//   gpr_driver -app bfslvl|cc|pr|pr_vary [options] <inFile>
+//   gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary [options] <inFile>
 //
 // Options follow the reference's names (driver.C:189-196, BFSLVLv.C:67-70,
 // CCv.C:175-178, PRv.C:73-77, PR_vary.C:197-201):
@@ -101,7 +101,7 @@ Args parse(int argc, char **argv) {
 }
 
 void usage() {
-    std::cerr << "usage: gpr_driver -app bfslvl|cc|pr|pr_vary [-b] [-s] [-rounds R] [-start V]\n"
+    std::cerr << "usage: gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary [-b] [-s] [-rounds R] [-start V]\n"
                  "                  [-pr:maxiter K] [-pr:outfile F] [-cc:outfile F] [-itimes]\n"
                  "                  [-rmat S [-ef E] [-seed X]] [-reorder] [-dir auto|push|pull|policy]\n"
                  "                  [-threshold T] [-fusion default|off|1|2] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
@@ -278,6 +278,45 @@ void validate_bfs(gpr_graph *g, uint32_t start, const std::vector<uint32_t> &lev
     std::cout << "Number of vertices: " << n << "\n";
 }
 
+// BFSv.C:205-230: no parent chain longer than the traversal's depth, every
+// vertex activated at most once.  Chain lengths are found with memoised
+// walks (linear time); a walk longer than n (a cycle) counts as too long.
+void validate_bfsv(gpr_graph *g, uint32_t start, int iters, const std::vector<uint32_t> &parent) {
+    const uint32_t n = (uint32_t)parent.size();
+    const uint32_t longest = iters > 0 ? (uint32_t)(iters - 1) : 0;
+    uint32_t start_int = 0;
+    CALL(gpr_graph_remap_id(g, start, &start_int));
+    std::cout << "Longest path from " << start_int << " (original: " << start << ") : " << longest << "\n";
+    // depth[v]: parent steps from v to its root (0 for the root and for
+    // unset vertices, as the reference's loop counts them); BAD for a chain
+    // that leaves the vertex range or runs longer than n (a cycle)
+    const uint32_t UNK = 0xffffffffu, BAD = 0xfffffffeu;
+    std::vector<uint32_t> depth(n, UNK), path;
+    uint64_t longer = 0, active = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        if (parent[v] != 0xffffffffu) ++active;
+        path.clear();
+        uint32_t p = v, d = 0;
+        for (;;) {
+            if (depth[p] != UNK) { d = depth[p]; break; }
+            const uint32_t q = parent[p];
+            if (q == 0xffffffffu || q == p) { depth[p] = d = 0; break; }
+            if (q >= n || path.size() >= n) { d = BAD; break; }
+            path.push_back(p);
+            p = q;
+        }
+        for (size_t i = path.size(); i-- > 0;) {
+            d = d == BAD ? BAD : d + 1;
+            depth[path[i]] = d;
+        }
+        if (depth[v] == BAD || depth[v] > longest) ++longer;
+    }
+    std::cout << "Number of vertices with longer chains: " << longer << (longer == 0 ? " PASS" : " FAIL") << "\n";
+    std::cout << "Number of activated vertices: " << active << "\n";
+    std::cout << "Number of vertices: " << n << "\n";
+    std::cout << "Every vertex activated at most once: " << (n >= active ? "PASS" : "FAIL") << "\n";
+}
+
 // CCv.C:604-660 with count_unique (bench/unique.h:44-128)
 void validate_cc(gpr_graph *g, const std::vector<uint32_t> &ids) {
     const uint32_t n = (uint32_t)ids.size();
@@ -328,7 +367,7 @@ template <class T> void write_lines(const std::string &path, const std::vector<T
 
 int run(const Args &a) {
     const std::string app = a.str("-app", "");
-    if (app != "bfslvl" && app != "cc" && app != "pr" && app != "pr_vary") {
+    if (app != "bfslvl" && app != "bfsv" && app != "cc" && app != "pr" && app != "pr_vary") {
         usage();
         return 2;
     }
@@ -382,12 +421,13 @@ int run(const Args &a) {
     const uint32_t start = (uint32_t)a.num("-start", 0);
     std::vector<gpr_pr_stage> stages = app == "pr" ? std::vector<gpr_pr_stage>{{GPR_CONTRIB_F32, 1e-7}}
                                                     : parse_stages(a.str("-pr:stages", "cfp16:1e-3,f32:1e-7"));
-    std::vector<uint32_t> out_u32(app == "bfslvl" || app == "cc" ? n : 0);
+    std::vector<uint32_t> out_u32(app == "bfslvl" || app == "bfsv" || app == "cc" ? n : 0);
     std::vector<float> out_f32(app == "pr" || app == "pr_vary" ? n : 0);
     double total = 0.0;
     for (long r = 0; r < rounds; ++r) {
         gpr_run_info info = {};
         if (app == "bfslvl") CALL(gpr_bfslvl(g, start, &o, out_u32.data(), &info));
+        else if (app == "bfsv") CALL(gpr_bfsv(g, start, &o, out_u32.data(), &info));
         else if (app == "cc") CALL(gpr_ccv(g, &o, out_u32.data(), &info));
         else CALL(gpr_pr_vary(g, stages.data(), (int)stages.size(), &o, out_f32.data(), &info));
         const double sec = info.ms * 1e-3;
@@ -404,6 +444,7 @@ int run(const Args &a) {
             write_lines(a.str("-pr:outfile", ""), out_f32, std::numeric_limits<float>::max_digits10);
         // validate
         if (app == "bfslvl") validate_bfs(g, start, out_u32, sym != 0);
+        else if (app == "bfsv") validate_bfsv(g, start, info.iters, out_u32);
         else if (app == "cc") validate_cc(g, out_u32);
         else {
             for (uint32_t i = 0; i < 10 && i < n; ++i) std::cout << "PR[" << i << "]: " << out_f32[i] << "\n";

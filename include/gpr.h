@@ -131,7 +131,10 @@ int gpr_frontier_destroy(gpr_frontier *f);
 enum {                      /* relax operator, api::relax (api.h:1670-1687) */
     GPR_RELAX_MIN_PLUS1 = 0, /* p[d].min(p[s]+1)   BFSLVLv.C:208-214 */
     GPR_RELAX_MIN = 1,       /* p[d].min(p[s])     CCv.C:571-577 */
-    GPR_RELAX_PR_SUM = 2     /* y[d] += contrib[s] PRv.C:202-212 */
+    GPR_RELAX_PR_SUM = 2,    /* y[d] += contrib[s] PRv.C:202-212 */
+    GPR_RELAX_SETIF = 3      /* p[d].setif(s): p[d] = s (the source's ID) where
+                                p[d] has its top bit set; changed once per
+                                vertex (BFSv.C:146, dsl/ast/redop.h:1396-1440) */
 };
 enum {                      /* destination filter, api::filter(dst) */
     GPR_DST_NONE = 0,
@@ -260,6 +263,12 @@ typedef struct {
 /* BFSLVLv (bench/BFSLVLv.C:110-254): levels[n], unreached = n+1. */
 int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *o,
                uint32_t *levels, gpr_run_info *info);
+/* BFSv (bench/BFSv.C:88-203): parents[n] (original IDs), the start its own
+ * parent, unreached = 0xffffffff.  Which frontier neighbour becomes a
+ * vertex's parent depends on the order of the relaxes, as in the reference;
+ * every parent is an in-neighbour one BFS level closer to the start. */
+int gpr_bfsv(gpr_graph *g, uint32_t start, const gpr_run_opts *o,
+             uint32_t *parents, gpr_run_info *info);
 /* CCv (bench/CCv.C:221-588): labels[n] = min vertex ID of the component
  * (the recode canonical form, CCv.C:121-169). */
 int gpr_ccv(gpr_graph *g, const gpr_run_opts *o, uint32_t *labels,

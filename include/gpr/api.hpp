@@ -210,6 +210,8 @@ struct relax_arg {
 inline relax_arg min_plus1(vertexprop<uint32_t> &p) { return {GPR_RELAX_MIN_PLUS1, p.get(), nullptr, 0, 0.f}; }
 // IDs[d].min( IDs[s] )
 inline relax_arg min_label(vertexprop<uint32_t> &p) { return {GPR_RELAX_MIN, p.get(), nullptr, 0, 0.f}; }
+// parent[d].setif(s) (BFSv.C:146)
+inline relax_arg setif(vertexprop<uint32_t> &p) { return {GPR_RELAX_SETIF, p.get(), nullptr, 0, 0.f}; }
 // y_acc[d] += cast<acc>( contrib[s] )
 template <class CT>
 inline relax_arg pr_sum(vertexprop<float> &y, vertexprop<CT> &contrib, int fmt, float damping = 0.85f) {
