@@ -5,7 +5,10 @@ One step = one BFSLVLv run (source: highest-degree vertex) + one CCv run +
 one PRv run of 20 FP32 power iterations, on a symmetric RMAT graph of scale
 26, edge factor 16 (a=.57 b=.19 c=.19 d=.05), generated and cleaned on the
 GPU before timing.  `value` is whole-job GTEPS: (BFS edges of the reached
-component + m for CC + 20*m for PR) / wall time of the K timed steps.
+component + m for CC + 20*m for PR) / wall time of the K timed steps.  The
+timed runs leave their results in HBM, as Graptor's driver times run()
+alone (driver.C:245-255); one untimed run afterwards downloads them for the
+parity check against the CPU run.
 
 Also reported: per-algorithm GTEPS, the HBM roofline of the dominant step
 (the PR edgemap: its kernels timed with HIP events on the engine's own
@@ -86,10 +89,13 @@ def run(args):
         g.reorder()
         log("degree-sorted relabelling (%.1fs)" % (time.time() - t0))
 
-    def step():
-        lv, ib = g.bfslvl(src)
-        lab, ic = g.ccv()
-        x, ip = g.prv(max_iter=args.pr_iters, tol=0.0)
+    # the timed runs leave their results on the device, as Graptor's driver
+    # times run() alone (driver.C:245-255); one untimed run afterwards
+    # downloads them for the parity check
+    def step(download=False):
+        lv, ib = g.bfslvl(src, download=download)
+        lab, ic = g.ccv(download=download)
+        x, ip = g.prv(max_iter=args.pr_iters, tol=0.0, download=download)
         return (lv, ib), (lab, ic), (x, ip)
 
     for w in range(args.warmup):
@@ -113,6 +119,7 @@ def run(args):
     wall = time.perf_counter() - t_start
     avg_pull_ms, per_scope, pull_launches = pr_edgemap_ms(g)
     g.profile(False)
+    (lv, _), (lab, _), (x, _) = step(download=True)
     total_edges = eb + ec + ep
     value = total_edges / wall / 1e9
     bytes_pull = pr_pull_bytes(n, m, 4)

@@ -286,43 +286,47 @@ class Graph:
         self.destroy()
 
     # ---- applications -------------------------------------------------------
-    def bfslvl(self, start, **kw):
-        out = np.empty(self.n, np.uint32)
+    # Each run returns its result array, or None with download=False: the
+    # result then stays on the device, as a benchmark's timed run() leaves it
+    # (driver.C:245-255 times run() only).
+    def bfslvl(self, start, download=True, **kw):
+        out = np.empty(self.n, np.uint32) if download else None
         o = opts(**kw)
         info = RunInfo()
-        _check(lib().gpr_bfslvl(self.h, start, C.byref(o), _ptr(out), C.byref(info)))
+        _check(lib().gpr_bfslvl(self.h, start, C.byref(o), _ptr(out) if download else None, C.byref(info)))
         return out, info
 
-    def bfsv(self, start, **kw):
+    def bfsv(self, start, download=True, **kw):
         """BFS parents (original IDs): the start its own parent, UNSET where
         unreached."""
-        out = np.empty(self.n, np.uint32)
+        out = np.empty(self.n, np.uint32) if download else None
         o = opts(**kw)
         info = RunInfo()
-        _check(lib().gpr_bfsv(self.h, start, C.byref(o), _ptr(out), C.byref(info)))
+        _check(lib().gpr_bfsv(self.h, start, C.byref(o), _ptr(out) if download else None, C.byref(info)))
         return out, info
 
-    def ccv(self, **kw):
-        out = np.empty(self.n, np.uint32)
+    def ccv(self, download=True, **kw):
+        out = np.empty(self.n, np.uint32) if download else None
         o = opts(**kw)
         info = RunInfo()
-        _check(lib().gpr_ccv(self.h, C.byref(o), _ptr(out), C.byref(info)))
+        _check(lib().gpr_ccv(self.h, C.byref(o), _ptr(out) if download else None, C.byref(info)))
         return out, info
 
-    def prv(self, max_iter=20, tol=0.0, **kw):
-        out = np.empty(self.n, np.float32)
+    def prv(self, max_iter=20, tol=0.0, download=True, **kw):
+        out = np.empty(self.n, np.float32) if download else None
         o = opts(max_iter=max_iter, tol=tol, **kw)
         info = RunInfo()
-        _check(lib().gpr_prv(self.h, C.byref(o), _ptr(out), C.byref(info)))
+        _check(lib().gpr_prv(self.h, C.byref(o), _ptr(out) if download else None, C.byref(info)))
         return out, info
 
-    def pr_vary(self, stages, max_iter=100, **kw):
+    def pr_vary(self, stages, max_iter=100, download=True, **kw):
         """stages: list of (format name or CONTRIB_*, tol)."""
         arr = (PrStage * len(stages))(*[PrStage(CONTRIB_NAMES.get(f, f), t) for f, t in stages])
-        out = np.empty(self.n, np.float32)
+        out = np.empty(self.n, np.float32) if download else None
         o = opts(max_iter=max_iter, **kw)
         info = RunInfo()
-        _check(lib().gpr_pr_vary(self.h, arr, len(stages), C.byref(o), _ptr(out), C.byref(info)))
+        _check(lib().gpr_pr_vary(self.h, arr, len(stages), C.byref(o), _ptr(out) if download else None,
+                                 C.byref(info)))
         return out, info
 
     def msbfs(self, sources, **kw):
