@@ -571,12 +571,15 @@ constexpr uint32_t nw_max = CA_THREADS / 64;
 // so hi = floor(t) and lo = trunc((t - hi) * 2^32) are the two 32-bit halves
 // of the integer.  (A result below 2^-126 may lose bits to a subnormal, but
 // its integer part is 0 either way.)
-__device__ __forceinline__ long long to_fixed(float v, int sh) {
+__device__ __forceinline__ uint64_t to_fixed_mag(float v, int sh) {
     const float t = ldexpf(fabsf(v), sh);
     const float h = floorf(t);
     const uint32_t hi = (uint32_t)h;
     const uint32_t lo = (uint32_t)ldexpf(t - h, 32);
-    const uint64_t mag = ((uint64_t)hi << 32) | lo;
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ long long to_fixed(float v, int sh) {
+    const uint64_t mag = to_fixed_mag(v, sh);
     return v < 0.0f ? -(long long)mag : (long long)mag;
 }
 
@@ -709,12 +712,18 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
             vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)vv[u]);
         }
         // LDS atomics are exact whatever the order and whatever the number
-        // of lanes on one address (the hub destinations)
+        // of lanes on one address (the hub destinations).  A block wholly
+        // inside the chunk needs no per-lane range test, and the sign is
+        // applied only in a wave with a negative value (PageRank's are not).
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            const uint32_t o = base + u * nw * 64 + ln;
-            const long long fx = to_fixed(C::dec(store_from_bits<S>(vb[u])) * dscale, shift);
-            if (o >= lo && o < len) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
+            const uint32_t ob = base + u * nw * 64, o = ob + ln;
+            const float v = C::dec(store_from_bits<S>(vb[u])) * dscale;
+            uint64_t fx = to_fixed_mag(v, shift);
+            if (__builtin_expect(__ballot(v < 0.0f) != 0, 0)) fx = v < 0.0f ? (uint64_t)(-(long long)fx) : fx;
+            bool in = true;
+            if (!(ob >= 64 && ob + 64 <= len)) in = o >= lo && o < len;
+            if (in) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
         }
     }
     __syncthreads();
