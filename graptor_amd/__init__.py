@@ -22,10 +22,13 @@ UNSET = 0xFFFFFFFF   # an unset BFSv parent
 DST_NONE, DST_GT, DST_NE = range(3)
 RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE = range(3)
 DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
-CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T = range(5)
+CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T, CONTRIB_SCFP16 = range(6)
 VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER = range(4)
 CONTRIB_NAMES = {"fp32": CONTRIB_F32, "bf16": CONTRIB_BF16, "fp16": CONTRIB_F16, "cfp16": CONTRIB_CFP16,
-                 "bf16t": CONTRIB_BF16T}
+                 "bf16t": CONTRIB_BF16T, "scfp16": CONTRIB_SCFP16}
+# APR_vary's default variant 2046: FP32 contributions down to 1e-1, then
+# scustomfp<true,6,9> down to 1e-7 (bench/APR_vary.C:37-39, :976-980).
+APR_STAGES_2046 = (("fp32", 1e-1), ("scfp16", 1e-7))
 
 
 class GprError(RuntimeError):
@@ -114,6 +117,8 @@ SIGNATURES = {
     "gpr_ccv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_prv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_pr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_apr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.c_double, C.POINTER(RunOpts), _P,
+                               C.POINTER(RunInfo)]),
     "gpr_msbfs": (C.c_int, [_P, _P, C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_batch_generate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _PP]),
     "gpr_batch_create": (C.c_int, [_P, _P, _P, C.c_uint32, _PP]),
@@ -327,6 +332,17 @@ class Graph:
         info = RunInfo()
         _check(lib().gpr_pr_vary(self.h, arr, len(stages), C.byref(o), _ptr(out) if download else None,
                                  C.byref(info)))
+        return out, info
+
+    def apr_vary(self, stages=APR_STAGES_2046, eps2=1e-2, max_iter=100, download=True, **kw):
+        """Delta PageRank (APR_vary): stages of (format name or CONTRIB_*, eps);
+        returns (x as FP64 scaled to unit L1 norm, RunInfo)."""
+        arr = (PrStage * len(stages))(*[PrStage(CONTRIB_NAMES.get(f, f), t) for f, t in stages])
+        out = np.empty(self.n, np.float64) if download else None
+        o = opts(max_iter=max_iter, **kw)
+        info = RunInfo()
+        _check(lib().gpr_apr_vary(self.h, arr, len(stages), eps2, C.byref(o), _ptr(out) if download else None,
+                                  C.byref(info)))
         return out, info
 
     def msbfs(self, sources, **kw):

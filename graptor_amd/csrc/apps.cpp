@@ -6,6 +6,7 @@
 //   CCv       bench/CCv.C:221-588      (run :221-471, emap_step :520-588)
 //   PRv       bench/PRv.C:115-290      (iteration :201-280)
 //   PR_vary   bench/PR_vary.C:247-757  (run_prec :247-447, staged run :652-757)
+//   APR_vary  bench/APR_vary.C:378-1074 (run_prec :378-640, run :900-1074)
 //
 // Each benchmark runs the reference's DEFERRED_UPDATE=0 configuration of
 // the record operator (api::record(output, api::reduction, api::strong),
@@ -13,6 +14,7 @@
 // its relax changed it.  For BFS levels and min-label CC this activates the
 // same vertices as the changed-vs-previous method and needs no prev copy.
 #include "../../include/gpr/api.hpp"
+#include "apr.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -339,7 +341,128 @@ struct PRv {
     }
 };
 
+// ---- APR_vary ----------------------------------------------------------------
+// Delta PageRank.  x and delta are FP32 throughout (FloatTy), z is the
+// edgemap's accumulator; the frontier, x and delta carry from one stage to
+// the next, and so does the iteration count.
+struct APRv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<float> x_mem, delta_mem, z_mem;
+    A::vertexprop<double> xn_mem;
+    APRv(gpr_graph *g, const gpr_run_opts &opts)
+        : GA(g), o(opts), x_mem(GA, "x"), delta_mem(GA, "delta"), z_mem(GA, "z"), xn_mem(GA, "x normalised") {}
+    ~APRv() {
+        x_mem.del();
+        delta_mem.del();
+        z_mem.del();
+        xn_mem.del();
+    }
+
+    // APR_vary.C:450-457 / :507 / :560 and frac_threshold(1) (api.h:136-139):
+    // 0 the first iteration, 1 every source, 2 the frontier's sources by
+    // pull, 3 by push.
+    int branch(int iter, uint64_t fv, uint64_t fe, uint64_t m) const {
+        if (iter == 0) return 0;
+        if (((double)fv + (double)fe) / (double)m > 0.6) return 1;
+        if (o.direction == GPR_DIR_PULL) return 2;
+        if (o.direction == GPR_DIR_PUSH) return 3;
+        return (double)(float)(fv + fe) <= (double)(float)m / 100.0 ? 3 : 2;
+    }
+
+    template <class CT>
+    void run_prec(int fmt, double eps, double eps2, float factor, int &iter, A::frontier &F, AprTotals &tot,
+                  gpr_run_info *info) {
+        A::vertexprop<CT> contrib(GA, "contrib");
+        const uint32_t n = GA.numVertices();
+        const uint64_t m = GA.numEdges();
+        while (iter < o.max_iter) {
+            const uint64_t fv = F.nActiveVertices(), fe = F.nActiveEdges();
+            const int kind = branch(iter, fv, fe, m);
+            gpr_iter_stats st = {};
+            st.nactv_in = kind <= 1 ? n : (uint32_t)fv;
+            st.nacte_in = kind <= 1 ? m : fe;
+            st.edges_scanned = kind <= 1 ? m : fe;
+            const auto t0 = std::chrono::steady_clock::now();
+            if (kind <= 2) {
+                // contributions, then z[d] += contrib[s] * factor over the
+                // CSC (the factor is applied once per vertex in the tail)
+                check(apr_contrib(GA.get(), fmt, delta_mem.get(), contrib.get(), factor,
+                                  kind == 2 ? F.get() : nullptr),
+                      "apr contributions");
+                auto lazy = A::edgemap(GA, A::relax(A::pr_sum(z_mem, contrib, fmt)));
+                lazy.materialize();
+                st.direction = GPR_DIR_PULL;
+            } else {
+                check(apr_push(GA.get(), fmt, F.get(), delta_mem.get(), factor, z_mem.get()), "apr push");
+                st.direction = GPR_DIR_PUSH;
+            }
+            A::frontier active = A::frontier::empty(GA);
+            check(apr_tail(GA.get(), kind == 0, z_mem.get(), factor, (float)eps2, x_mem.get(), delta_mem.get(),
+                           active.get(), &tot),
+                  "apr vertex maps");
+            st.nactv_out = (uint32_t)tot.nactv;
+            st.nacte_out = tot.nacte;
+            st.ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            record_step(GA.get(), o, st, info);
+            info->edges += st.edges_scanned;
+            ++iter;
+            F.del();
+            F = active;
+            if (tot.l1 < eps) break;
+        }
+        contrib.del();
+    }
+
+    void run(const gpr_pr_stage *stages, int nstages, double eps2, gpr_run_info *info) {
+        const uint32_t n = GA.numVertices();
+        for (int s = 0; s < nstages; ++s) check(apr_check_fmt(stages[s].contrib), "apr_vary stage");
+        // APR_vary.C:436-448: factor = 10^-(decimal digits of n), in FP32
+        float factor = 1.0f;
+        for (uint32_t nnn = n; nnn > 1; nnn /= 10) factor = (float)((double)factor / 10.0);
+        // :917-921 and :426-430: x = 0, delta = 1/n
+        A::make_lazy_executor(GA)
+            .vertex_map_fill(x_mem, 0.0f)
+            .vertex_map_fill(delta_mem, (float)(1.0 / (double)n))
+            .materialize();
+        A::frontier F = A::frontier::all_true(GA);
+        AprTotals tot = {};
+        int iter = 0;
+        for (int s = 0; s < nstages; ++s) {
+            if (stages[s].contrib == GPR_CONTRIB_F32)
+                run_prec<float>(stages[s].contrib, stages[s].tol, eps2, factor, iter, F, tot, info);
+            else
+                run_prec<uint16_t>(stages[s].contrib, stages[s].tol, eps2, factor, iter, F, tot, info);
+        }
+        F.del();
+        // :1060-1072: x scaled to unit L1 norm, in FP64
+        info->delta = tot.l1;
+        info->sum = tot.sum_x;
+        check(apr_scale(GA.get(), x_mem.get(), tot.sum_x > 0.0 ? 1.0 / tot.sum_x : 0.0, xn_mem.get()),
+              "apr scale");
+    }
+};
+
 } // namespace
+
+extern "C" int gpr_apr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages, double eps2,
+                            const gpr_run_opts *opts, double *x, gpr_run_info *info) {
+    if (!g || !stages || nstages <= 0 || !(eps2 >= 0.0)) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    uint32_t n = 0;
+    gpr_graph_info(g, &n, nullptr, nullptr);
+    if (n == 0) return guarded([&] { check(GPR_E_INVAL, "apr_vary: empty graph"); });
+    return guarded([&] {
+        APRv app(g, o);
+        gpr_run_info local = {};
+        if (o.trace) trace_clear(g);
+        stream_time_ms(g, true);
+        app.run(stages, nstages, eps2, &local);
+        local.ms = stream_time_ms(g, false);
+        if (x) app.xn_mem.download(x);
+        if (info) *info = local;
+    });
+}
 
 extern "C" int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *opts, uint32_t *levels,
                           gpr_run_info *info) {

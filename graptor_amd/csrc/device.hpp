@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "internal.hpp"
+
 namespace gpr {
 
 __device__ __forceinline__ unsigned lane() { return __lane_id(); }
@@ -64,6 +66,22 @@ __device__ __forceinline__ bool dst_ok(uint32_t v, int kind, uint32_t arg) {
 
 __device__ __forceinline__ bool bit_test(const uint64_t *bits, uint32_t v) {
     return (bits[v >> 6] >> (v & 63)) & 1ull;
+}
+
+// Edge position of frontier entry i from the blocked degree prefix
+// (scan[i] + scan_blk[i / SCAN_BLOCK]).
+__device__ __forceinline__ uint64_t frontier_pos(const uint64_t *pos, const uint64_t *btot, uint32_t i) {
+    return pos[i] + btot[i / SCAN_BLOCK];
+}
+
+// First index i in [0,k) with pos(i) > e (k if none).
+__device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *btot, uint32_t k, uint64_t e) {
+    uint32_t lo = 0, hi = k;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (frontier_pos(pos, btot, mid) <= e) lo = mid + 1; else hi = mid;
+    }
+    return lo;
 }
 
 } // namespace gpr

@@ -87,20 +87,6 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint64_t *a, uint32_t count) 
     if (threadIdx.x == 0) a[count] = carry;
 }
 
-__device__ __forceinline__ uint64_t frontier_pos(const uint64_t *pos, const uint64_t *btot, uint32_t i) {
-    return pos[i] + btot[i / SCAN_BLOCK];
-}
-
-// First index i in [0,k) with pos(i) > e (k if none).
-__device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *btot, uint32_t k, uint64_t e) {
-    uint32_t lo = 0, hi = k;
-    while (lo < hi) {
-        uint32_t mid = (lo + hi) >> 1;
-        if (frontier_pos(pos, btot, mid) <= e) lo = mid + 1; else hi = mid;
-    }
-    return lo;
-}
-
 template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) {
     return RELAX == GPR_RELAX_MIN_PLUS1 ? ps + 1u : ps;
 }

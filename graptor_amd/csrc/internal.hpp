@@ -254,6 +254,7 @@ int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_fr
 // PageRank cold-edge layout (pr_cold.hip)
 int cold_prepare(gpr_graph *g, uint32_t from);   // (re)build for this split point
 void cold_free(gpr_graph *g);
+int scan_frontier_degrees(gpr_graph *g, const uint32_t *list, uint32_t k);
 int cold_launch_write(gpr_graph *g, int fmt, const void *contrib);
 // y (+)= the binned sums; *nparts receives the number of y partial slots
 int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, float *part_max, uint32_t *nparts);

@@ -379,7 +379,7 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
     GPR_REQUIRE(!in || in->kind == GPR_FRONTIER_ALL, "PR edgemap runs over the all-true frontier only");
     GPR_REQUIRE(desc->prop && desc->prop->dtype == GPR_F32, "PR edgemap: accumulator must be an f32 property");
     const int fmt = desc->contrib_fmt;
-    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_BF16T, "PR edgemap: unknown contribution format %d", fmt);
+    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_SCFP16, "PR edgemap: unknown contribution format %d", fmt);
     GPR_REQUIRE(desc->contrib && desc->contrib->elem == fmt_bytes(fmt),
                 "PR edgemap: contribution property has the wrong element size for format %d", fmt);
     uint32_t cold_from = split_point(g);
@@ -416,6 +416,7 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
         case GPR_CONTRIB_F16: PR_PULL(GPR_CONTRIB_F16); break;
         case GPR_CONTRIB_CFP16: PR_PULL(GPR_CONTRIB_CFP16); break;
         case GPR_CONTRIB_BF16T: PR_PULL(GPR_CONTRIB_BF16T); break;
+        case GPR_CONTRIB_SCFP16: PR_PULL(GPR_CONTRIB_SCFP16); break;
         }
 #undef PR_PULL
     }
@@ -460,7 +461,7 @@ using namespace gpr;
 
 extern "C" int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib, int fmt, float damping) {
     if (!g || !x || !contrib || x->dtype != GPR_F32) return GPR_E_INVAL;
-    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_BF16T && contrib->elem == fmt_bytes(fmt),
+    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_SCFP16 && contrib->elem == fmt_bytes(fmt),
                 "pr_contrib: bad contribution format %d / property", fmt);
     const uint32_t n = g->n;
     const unsigned grid = update_grid(n);
@@ -484,6 +485,7 @@ extern "C" int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib
     case GPR_CONTRIB_F16: PR_CONTRIB(GPR_CONTRIB_F16); break;
     case GPR_CONTRIB_CFP16: PR_CONTRIB(GPR_CONTRIB_CFP16); break;
     case GPR_CONTRIB_BF16T: PR_CONTRIB(GPR_CONTRIB_BF16T); break;
+    case GPR_CONTRIB_SCFP16: PR_CONTRIB(GPR_CONTRIB_SCFP16); break;
     }
 #undef PR_CONTRIB
     GPR_HIP(hipGetLastError());
@@ -493,7 +495,7 @@ extern "C" int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib
 extern "C" int gpr_pr_finish(gpr_graph *g, gpr_prop *x, const gpr_prop *y, gpr_prop *contrib, int fmt, float damping,
                              double *delta, double *sum_x) {
     if (!g || !x || !y || !contrib || x->dtype != GPR_F32 || y->dtype != GPR_F32) return GPR_E_INVAL;
-    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_BF16T && contrib->elem == fmt_bytes(fmt),
+    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_SCFP16 && contrib->elem == fmt_bytes(fmt),
                 "pr_finish: bad contribution format %d / property", fmt);
     const uint32_t n = g->n;
     const unsigned grid = update_grid(n);
@@ -510,6 +512,7 @@ extern "C" int gpr_pr_finish(gpr_graph *g, gpr_prop *x, const gpr_prop *y, gpr_p
         case GPR_CONTRIB_F16: PR_UPDATE(GPR_CONTRIB_F16); break;
         case GPR_CONTRIB_CFP16: PR_UPDATE(GPR_CONTRIB_CFP16); break;
         case GPR_CONTRIB_BF16T: PR_UPDATE(GPR_CONTRIB_BF16T); break;
+        case GPR_CONTRIB_SCFP16: PR_UPDATE(GPR_CONTRIB_SCFP16); break;
         }
 #undef PR_UPDATE
         hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, pd, g->partial_f, psum, grid,

@@ -2,8 +2,9 @@
 //
 // FP32 (PRv, accumulated in FP64 as PRv.C:127 does), bfloat16 with
 // round-to-nearest-even, bfloat16 by truncation (scustomfp<true,8,7>),
-// IEEE half with a per-iteration power-of-two scale, and customfp<6,10>
-// (customfp.h:95-117 encode, :141-169 decode).
+// IEEE half with a per-iteration power-of-two scale, customfp<6,10>
+// (customfp.h:95-117 encode, :141-169 decode) and the signed
+// scustomfp<true,6,9,true,0> of the delta PageRank (APR_vary.C:743-750).
 #pragma once
 #include "internal.hpp"
 
@@ -60,6 +61,24 @@ template <> struct Codec<GPR_CONTRIB_CFP16> {
     using acc_t = float;
     __device__ static float dec(uint16_t h) { return __uint_as_float(((uint32_t)h | 0x10000u) << 13); }
     __device__ static uint16_t enc(float f) { return (uint16_t)(__float_as_uint(f) >> 13); }
+};
+// scustomfp<true,6,9,true,0>: sign, the low 6 exponent bits and the top 9
+// mantissa bits, truncated (customfp.h:95-117; the vector narrowing of
+// target/conversion.h:467-497 gives the same bits).  Decode puts exponent
+// bits 01 above the six, so magnitudes cover [2^-63, 2), and code 0 decodes
+// to 0 as the vector widening does for a format that may hold zero
+// (target/conversion.h:150-189).
+template <> struct Codec<GPR_CONTRIB_SCFP16> {
+    using store_t = uint16_t;
+    using acc_t = float;
+    __device__ static float dec(uint16_t h) {
+        const uint32_t t = ((uint32_t)(h & 0x8000u) << 16) | 0x20000000u | ((uint32_t)(h & 0x7fffu) << 14);
+        return h ? __uint_as_float(t) : 0.0f;
+    }
+    __device__ static uint16_t enc(float f) {
+        const uint32_t u = __float_as_uint(f);
+        return (uint16_t)(((u >> 14) & 0x7fffu) | ((u >> 16) & 0x8000u));
+    }
 };
 
 } // namespace gpr

@@ -792,6 +792,7 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
     case GPR_CONTRIB_F16: CW(GPR_CONTRIB_F16); break;
     case GPR_CONTRIB_CFP16: CW(GPR_CONTRIB_CFP16); break;
     case GPR_CONTRIB_BF16T: CW(GPR_CONTRIB_BF16T); break;
+    case GPR_CONTRIB_SCFP16: CW(GPR_CONTRIB_SCFP16); break;
     }
 #undef CW
     GPR_HIP(hipGetLastError());
@@ -810,6 +811,7 @@ int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, fl
     case GPR_CONTRIB_F16: CA(GPR_CONTRIB_F16); break;
     case GPR_CONTRIB_CFP16: CA(GPR_CONTRIB_CFP16); break;
     case GPR_CONTRIB_BF16T: CA(GPR_CONTRIB_BF16T); break;
+    case GPR_CONTRIB_SCFP16: CA(GPR_CONTRIB_SCFP16); break;
     }
 #undef CA
     if (L->nmulti)

@@ -157,7 +157,9 @@ enum {                      /* contribution storage for GPR_RELAX_PR_SUM */
     GPR_CONTRIB_BF16 = 1,    /* bfloat16 RNE store, FP32 accumulate */
     GPR_CONTRIB_F16 = 2,     /* IEEE half + per-iteration 2^k scale */
     GPR_CONTRIB_CFP16 = 3,   /* customfp<6,10> truncating (customfp.h) */
-    GPR_CONTRIB_BF16T = 4    /* scustomfp<true,8,7>: bf16 truncating */
+    GPR_CONTRIB_BF16T = 4,   /* scustomfp<true,8,7>: bf16 truncating */
+    GPR_CONTRIB_SCFP16 = 5   /* scustomfp<true,6,9,true,0>: signed, truncating
+                                (APR_vary.C:743-750); code 0 reads as 0 */
 };
 
 typedef struct {
@@ -282,6 +284,23 @@ int gpr_prv(gpr_graph *g, const gpr_run_opts *o, float *x,
 typedef struct { int contrib; double tol; } gpr_pr_stage;
 int gpr_pr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages,
                 const gpr_run_opts *o, float *x, gpr_run_info *info);
+
+/* APR_vary (bench/APR_vary.C:378-640, :900-1074): delta PageRank.  The
+ * stages of (contribution format, eps) run in order from x = 0,
+ * delta = 1/n, sharing max_iter (-pr:maxiter), the frontier, x and delta;
+ * a stage ends when sum |delta| < eps.  eps2 is the relative activity
+ * threshold (the reference's 1e-2): a vertex stays active while
+ * |delta| > eps2 * x.  x[n] receives the ranks scaled to unit L1 norm
+ * (FP64, original IDs).  Formats: GPR_CONTRIB_F32 (variant 20, FP64
+ * accumulation), GPR_CONTRIB_SCFP16 (variant 46), GPR_CONTRIB_BF16,
+ * GPR_CONTRIB_BF16T; the unsigned formats are rejected.  Variant 2046 is
+ * {F32, 1e-1} then {SCFP16, 1e-7}.  o->direction: GPR_DIR_AUTO follows the
+ * reference (every source above 60% density, push at or below 1% of the
+ * edges); GPR_DIR_PUSH / GPR_DIR_PULL force the frontier iterations.
+ * info->delta is the last sum |delta|, info->sum the sum |x| before the
+ * scaling. */
+int gpr_apr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages, double eps2,
+                 const gpr_run_opts *o, double *x, gpr_run_info *info);
 
 /* Multi-source BFSLVLv: k <= 64 sources (original IDs) as the bit lanes of
  * one traversal; levels[j*n + v] is the level of v from sources[j] (n+1

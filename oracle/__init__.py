@@ -65,6 +65,14 @@ def lib():
         L.orc_graptor_pr.restype = C.c_int
         L.orc_graptor_pr.argtypes = [C.c_uint32, u64p, u32p, u32p, C.c_int, C.c_double, f32p, C.c_int,
                                      C.POINTER(C.c_double)]
+        L.orc_scfp16_encode.restype = C.c_uint16
+        L.orc_scfp16_encode.argtypes = [C.c_float]
+        L.orc_scfp16_decode.restype = C.c_float
+        L.orc_scfp16_decode.argtypes = [C.c_uint16]
+        L.orc_apr_vary.restype = C.c_int
+        L.orc_apr_vary.argtypes = [C.c_uint32, C.c_uint64, u64p, u32p, u32p, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, f64p, C.c_int,
+                                   C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_void_p]
         L.orc_cfp16_encode.restype = C.c_uint16
         L.orc_cfp16_encode.argtypes = [C.c_float]
         L.orc_cfp16_decode.restype = C.c_float
@@ -283,6 +291,41 @@ def bf16_decode(h):
 # --------------------------------------------------------------------------
 # numpy restatements for small cases
 # --------------------------------------------------------------------------
+def scfp16_decode_c(h):
+    return lib().orc_scfp16_decode(int(h))
+
+
+def scfp16_encode_c(f):
+    return lib().orc_scfp16_encode(float(f))
+
+
+APR_STAGES_2046 = ((0, 1e-1), (5, 1e-7))
+
+
+def apr_vary(index, nbrs, stages=APR_STAGES_2046, max_iter=100, eps2=1e-2, threads=None, outdeg=None,
+             cindex=None, cnbrs=None):
+    """Delta PageRank (APR_vary).  Returns (x normalised, iterations, final
+    L1, edges traversed, trace[iters, 4] = kind, nactv, nacte, L1)."""
+    n = len(index) - 1
+    if outdeg is None:
+        outdeg = degrees(index)
+    if cindex is None:
+        cindex, cnbrs = index, nbrs
+    fm = (C.c_int * len(stages))(*[int(f) for f, _ in stages])
+    ep = (C.c_double * len(stages))(*[float(e) for _, e in stages])
+    x = np.empty(max(n, 1), np.float64)
+    l1 = C.c_double()
+    ed = C.c_uint64()
+    tr = np.zeros((max_iter, 4), np.float64)
+    it = lib().orc_apr_vary(n, len(cnbrs), np.ascontiguousarray(cindex, np.uint64),
+                            np.ascontiguousarray(cnbrs, np.uint32), np.ascontiguousarray(outdeg, np.uint32),
+                            fm, ep, len(stages), max_iter, eps2, x, threads or os.cpu_count(), C.byref(l1),
+                            C.byref(ed), tr.ctypes.data)
+    if it < 0:
+        raise ValueError("apr_vary: contribution format without a sign")
+    return x[:n], it, l1.value, ed.value, tr[:it]
+
+
 def pr_dense_fp64(index, nbrs, iters, damping=0.85):
     """Dense-matrix power iteration of the same update (tiny graphs only):
     P[d, s] = 1/outdeg(s) for each edge s->d; y = d*P*x; y += (1-sum y)/n;

@@ -574,3 +574,174 @@ uint16_t orc_bf16_rne_encode(float f) {
     return (uint16_t)(u >> 16);
 }
 float orc_bf16_decode(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+/* scustomfp<true,6,9,true,0> (bench/APR_vary.C:743-750, variant 46):
+ * encode keeps the sign, the low 6 exponent bits and the top 9 mantissa
+ * bits, truncating (customfp.h:95-117; the vector narrowing,
+ * target/conversion.h:467-497, gives the same bits).  Decode restores the
+ * exponent bits 01 above the six (customfp.h:141-169) and, as the vector
+ * widening does for a format that may hold zero (target/conversion.h:
+ * 150-189), decodes code 0 to 0. */
+uint16_t orc_scfp16_encode(float f) {
+    uint32_t u = f2u(f);
+    return (uint16_t)(((u >> 14) & 0x7fffu) | ((u >> 16) & 0x8000u));
+}
+float orc_scfp16_decode(uint16_t h) {
+    if (h == 0) return 0.0f;
+    uint32_t t = ((uint32_t)(h & 0x8000u) << 16) | 0x20000000u | ((uint32_t)(h & 0x7fffu) << 14);
+    return u2f(t);
+}
+
+/* Contribution formats of the delta PageRank: FP32 (accumulated in FP64),
+ * bfloat16 RNE, bfloat16 truncating, scustomfp<true,6,9> (FP32
+ * accumulation).  Returns -1 for formats that cannot hold a sign. */
+static int apr_fmt_ok(int fmt) { return fmt == 0 || fmt == 1 || fmt == 4 || fmt == 5; }
+static uint32_t apr_enc(int fmt, float v) {
+    switch (fmt) {
+    case 0: return f2u(v);
+    case 1: return orc_bf16_rne_encode(v);
+    case 4: return orc_bf16t_encode(v);
+    default: return orc_scfp16_encode(v);
+    }
+}
+static float apr_dec(int fmt, uint32_t c) {
+    switch (fmt) {
+    case 0: return u2f(c);
+    case 1:
+    case 4: return orc_bf16_decode((uint16_t)c);
+    default: return orc_scfp16_decode((uint16_t)c);
+    }
+}
+
+/* APR_vary (bench/APR_vary.C): delta PageRank.  run() (:900-1074) starts
+ * from x = 0, delta = 1/n and runs the stages in order (variant 2046 is
+ * {FP32 contributions with FP64 accumulation, eps 1e-1} then
+ * {scustomfp<true,6,9>, eps 1e-7}), all sharing max_iter, the frontier,
+ * x and delta; the result is x scaled to unit L1 norm (:1060-1072).
+ * One stage is run_prec (:378-640):
+ *   factor = 10^-(digits of n) in FP32 (:436-448), outdeg = 1/(factor*deg);
+ *   iteration 0 (:459-505): contrib = delta*outdeg over all vertices,
+ *     z = sum over in-edges of contrib*factor, delta = d*z + (1-d)/n,
+ *     x += delta, delta -= 1/n, active = |delta| > eps2*x;
+ *   later iterations: over every source when the frontier's density
+ *     (nactv+nacte)/m exceeds 0.6 (:507-558), over the frontier's sources
+ *     only otherwise (:560-612; frac_threshold(1) picks push at <= 1%),
+ *     delta = z*d, active = |delta| > eps2*x, x += delta where active;
+ *   L1 = sum |delta| (:632-639), frontier = active, stop when L1 < eps.
+ * Every vertex operation is FP32 as FloatTy is; z accumulates in FP64 for
+ * FP32 contributions (variant 20) and FP32 otherwise (variant 46), in CSC
+ * order; L1 is summed in FP64.  trace (4 doubles per iteration, may be
+ * NULL): kind (0 first, 1 all sources, 2 frontier sources by pull, 3 by
+ * push), nactv, nacte of the new frontier, L1. */
+int orc_apr_vary(uint32_t n, uint64_t m, const uint64_t *cindex, const uint32_t *cnbrs,
+                 const uint32_t *outdeg, const int *fmts, const double *eps, int nstages,
+                 int max_iter, double eps2, double *x_out, int threads, double *l1_out,
+                 uint64_t *edges_out, double *trace) {
+    for (int s = 0; s < nstages; ++s)
+        if (!apr_fmt_ok(fmts[s])) return -1;
+    if (n == 0) return 0;
+    float *x = (float *)malloc(sizeof(float) * n);
+    float *delta = (float *)malloc(sizeof(float) * n);
+    float *zf = (float *)malloc(sizeof(float) * n);
+    uint32_t *cb = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint8_t *act = (uint8_t *)malloc(n);
+    uint8_t *nact = (uint8_t *)malloc(n);
+    const float dampen = 0.85f;
+    float factor = 1.0f;
+    for (uint32_t nnn = n; nnn > 1; nnn /= 10) factor = (float)((double)factor / 10.0);
+    const float ninv = 1.0f / (float)n;
+    const float cst = (1.0f - dampen) * ninv;
+    const float e2 = (float)eps2;
+    const float d0 = (float)(1.0 / (double)n);
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t v = 0; v < (int64_t)n; ++v) {
+        x[v] = 0.0f;
+        delta[v] = d0;
+        act[v] = 1;
+    }
+    uint64_t fv = n, fe = m, edges = 0;
+    int iter = 0;
+    double l1 = 0.0;
+    for (int st = 0; st < nstages; ++st) {
+        const int fmt = fmts[st];
+        while (iter < max_iter) {
+            int kind;
+            if (iter == 0) kind = 0;
+            else if (((double)fv + (double)fe) / (double)m > 0.6) kind = 1;
+            else kind = (double)(float)(fv + fe) <= (double)(float)m / 100.0 ? 3 : 2;
+            const int all = kind <= 1;
+            edges += all ? m : fe;
+#pragma omp parallel for num_threads(threads) schedule(static)
+            for (int64_t v = 0; v < (int64_t)n; ++v)
+                if ((all || act[v]) && outdeg[v]) {
+                    const float od = 1.0f / (factor * (float)outdeg[v]);
+                    cb[v] = apr_enc(fmt, delta[v] * od);
+                }
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 4096)
+            for (int64_t d = 0; d < (int64_t)n; ++d) {
+                if (fmt == 0) {
+                    double acc = 0.0;
+                    for (uint64_t e = cindex[d]; e < cindex[d + 1]; ++e) {
+                        const uint32_t s = cnbrs[e];
+                        if (all || act[s]) acc += (double)apr_dec(fmt, cb[s]) * (double)factor;
+                    }
+                    zf[d] = (float)acc;
+                } else {
+                    float acc = 0.0f;
+                    for (uint64_t e = cindex[d]; e < cindex[d + 1]; ++e) {
+                        const uint32_t s = cnbrs[e];
+                        if (all || act[s]) acc += apr_dec(fmt, cb[s]) * factor;
+                    }
+                    zf[d] = acc;
+                }
+            }
+            double sum = 0.0;
+            uint64_t cv = 0, ce = 0;
+#pragma omp parallel for num_threads(threads) schedule(static) reduction(+ : sum, cv, ce)
+            for (int64_t v = 0; v < (int64_t)n; ++v) {
+                float dl;
+                int a;
+                if (kind == 0) {
+                    dl = dampen * zf[v] + cst;
+                    x[v] = x[v] + dl;
+                    dl = dl - ninv;
+                    a = fabsf(dl) > e2 * x[v];
+                } else {
+                    dl = zf[v] * dampen;
+                    a = fabsf(dl) > e2 * x[v];
+                    if (a) x[v] = x[v] + dl;
+                }
+                delta[v] = dl;
+                nact[v] = (uint8_t)a;
+                sum += (double)fabsf(dl);
+                cv += (uint64_t)a;
+                ce += a ? outdeg[v] : 0;
+            }
+            memcpy(act, nact, n);
+            fv = cv;
+            fe = ce;
+            l1 = sum;
+            if (trace) {
+                trace[4 * iter] = kind;
+                trace[4 * iter + 1] = (double)cv;
+                trace[4 * iter + 2] = (double)ce;
+                trace[4 * iter + 3] = sum;
+            }
+            ++iter;
+            if (l1 < eps[st]) break;
+        }
+    }
+    double ss = 0.0;
+    for (uint32_t v = 0; v < n; ++v) ss += fabs((double)x[v]);
+    const double sc = 1.0 / ss;
+    for (uint32_t v = 0; v < n; ++v) x_out[v] = (double)x[v] * sc;
+    free(x);
+    free(delta);
+    free(zf);
+    free(cb);
+    free(act);
+    free(nact);
+    if (l1_out) *l1_out = l1;
+    if (edges_out) *edges_out = edges;
+    return iter;
+}

@@ -116,6 +116,19 @@ uint16_t orc_bf16t_encode(float f);
 uint16_t orc_bf16_rne_encode(float f);
 float orc_bf16_decode(uint16_t h);
 
+/* scustomfp<true,6,9,true,0>: sign, low 6 exponent bits, top 9 mantissa
+ * bits, truncating; code 0 decodes to 0 (bench/APR_vary.C:743-750). */
+uint16_t orc_scfp16_encode(float f);
+float orc_scfp16_decode(uint16_t h);
+
+/* APR_vary (bench/APR_vary.C:378-640, :900-1074): delta PageRank over
+ * stages of (contribution format, eps); x_out is scaled to unit L1 norm.
+ * Returns the iteration count, -1 for a format without a sign. */
+int orc_apr_vary(uint32_t n, uint64_t m, const uint64_t *cindex, const uint32_t *cnbrs,
+                 const uint32_t *outdeg, const int *fmts, const double *eps, int nstages,
+                 int max_iter, double eps2, double *x_out, int threads, double *l1_out,
+                 uint64_t *edges_out, double *trace);
+
 #ifdef __cplusplus
 }
 #endif

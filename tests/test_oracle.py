@@ -20,6 +20,15 @@ each oracle function is pinned by mathematics instead:
   orc_cfp16_*          exhaustive 2^16 decode against the closed form
                        (1 + m/1024) * 2^(e-63), truncation of every encode
   orc_bf16*            truncation vs the upper half; RNE vs torch.bfloat16
+  orc_scfp16_*         exhaustive 2^16 decode against the closed form
+                       (-1)^s (1 + m/512) 2^(e-63) (code 0 -> 0), truncation
+                       toward zero of every encode, the exponent aliasing
+  orc_apr_vary         with eps2 = 0 the delta iteration equals the FP64
+                       dense power iteration p' = d*P*p + (1-d)/n after the
+                       same number of iterations (directed graphs with sinks
+                       and sources, CSR != CSC); a pure-Python FP32 brute
+                       force of the frontier branches on tiny graphs, bit for
+                       bit; the default stages converge near PageRank
 """
 import os
 import struct
@@ -342,3 +351,208 @@ def test_batch_sizes_and_graphs():
     src = np.repeat(np.arange(len(index) - 1), np.diff(index).astype(np.int64))
     g_of = np.searchsorted(voff, src, side="right") - 1
     assert (nbrs >= voff[g_of]).all() and (nbrs < voff[g_of + 1]).all()    # no cross-graph edge
+
+
+# ---- delta PageRank (APR_vary) ---------------------------------------------------
+def scfp16_closed(h):
+    if h == 0:
+        return 0.0
+    sg = -1.0 if h & 0x8000 else 1.0
+    return sg * float(np.ldexp(1.0 + (h & 511) / 512.0, ((h >> 9) & 63) - 63))
+
+
+def test_scfp16_exhaustive_closed_form():
+    L = O.lib()
+    for h in range(1 << 16):
+        v = L.orc_scfp16_decode(h)
+        assert v == scfp16_closed(h)
+        if h:
+            assert L.orc_scfp16_encode(v) == h
+    assert L.orc_scfp16_encode(0.0) == 0
+    # truncation toward zero inside the range [2^-63, 2), sign kept
+    rng = np.random.default_rng(11)
+    mag = np.ldexp(rng.random(20000) + 1.0, rng.integers(-63, 1, 20000))
+    xs = (mag * np.where(rng.random(20000) < 0.5, -1.0, 1.0)).astype(np.float32)
+    for x in xs:
+        h = L.orc_scfp16_encode(float(x))
+        d = L.orc_scfp16_decode(h)
+        assert (d < 0) == (x < 0)
+        assert abs(d) <= abs(x)
+        if (h & 0x7FFF) != 0x7FFF:
+            assert abs(x) < abs(L.orc_scfp16_decode(h + 1))
+    # magnitudes >= 2 keep only their low exponent bits (no saturation)
+    assert L.orc_scfp16_encode(2.0) == 0
+    assert L.orc_scfp16_decode(L.orc_scfp16_encode(3.0)) == scfp16_closed(0x100)
+
+
+def directed_graph(n, m, seed):
+    """Random directed graph with sinks and sources: CSR, CSC, out-degrees."""
+    rng = np.random.default_rng(seed)
+    src = rng.integers(0, n, m).astype(np.uint32)
+    dst = rng.integers(0, n, m).astype(np.uint32)
+    keep = (src % 7 != 3) & (dst % 11 != 5)   # sinks: v%7==3, sources: v%11==5
+    src, dst = src[keep], dst[keep]
+    index, nbrs = O.build_csr(n, src, dst, False)
+    cindex, cnbrs = O.build_csr(n, dst, src, False)
+    return index, nbrs, cindex, cnbrs, O.degrees(index)
+
+
+def power_iterate(index, nbrs, iters, d=0.85):
+    n = len(index) - 1
+    deg = np.diff(index).astype(np.float64)
+    P = np.zeros((n, n))
+    for s in range(n):
+        for e in range(int(index[s]), int(index[s + 1])):
+            P[int(nbrs[e]), s] += 1.0 / deg[s]
+    p = np.full(n, 1.0 / n)
+    for _ in range(iters):
+        p = d * (P @ p) + (1.0 - d) / n
+    return p
+
+
+@pytest.mark.parametrize("graph", ["rmat", "directed"])
+# scustomfp<true,6,9> holds magnitudes below 2 only; from iteration 0 the
+# scaled contributions 1/(n*factor*deg) lie in (1, 10] for small degrees, so
+# (as in the reference's variant 2046) it runs after an FP32 stage.
+@pytest.mark.parametrize("stages,tol", [(((0, 0.0),), 1e-6), (((4, 0.0),), 5e-3), (((0, 0.1), (5, 0.0)), 1e-3)])
+def test_apr_equals_power_iteration(graph, stages, tol):
+    if graph == "rmat":
+        index, nbrs = O.rmat_graph(8)
+        cindex, cnbrs, deg = index, nbrs, O.degrees(index)
+    else:
+        index, nbrs, cindex, cnbrs, deg = directed_graph(300, 1500, 4)
+    K = 12
+    x, it, l1, ed, tr = O.apr_vary(index, nbrs, stages=stages, max_iter=K, eps2=0.0, outdeg=deg,
+                                   cindex=cindex, cnbrs=cnbrs, threads=2)
+    assert it == K
+    p = power_iterate(index, nbrs, K)
+    assert np.abs(x - p / np.abs(p).sum()).sum() < tol
+    # a transposed graph is a different operator: the pin sees it
+    if graph == "directed":
+        xt, *_ = O.apr_vary(cindex, cnbrs, stages=stages, max_iter=K, eps2=0.0, outdeg=O.degrees(cindex),
+                            cindex=index, cnbrs=nbrs, threads=2)
+        assert np.abs(xt - p / np.abs(p).sum()).sum() > 10 * tol
+
+
+def _enc(fmt, v):
+    v = np.float32(v)
+    u = int(np.array([v], np.float32).view(np.uint32)[0])
+    if fmt == 0:
+        return v
+    return ((u >> 14) & 0x7FFF) | ((u >> 16) & 0x8000)
+
+
+def _dec(fmt, c):
+    if fmt == 0:
+        return np.float32(c)
+    return np.float32(scfp16_closed(c))
+
+
+def apr_brute(index, nbrs, stages, max_iter, eps2):
+    """Pure-Python FP32 restatement of APR_vary.C:436-640 / :900-1074."""
+    f32 = np.float32
+    n = len(index) - 1
+    m = int(index[-1])
+    deg = [int(index[v + 1] - index[v]) for v in range(n)]
+    ins = [[] for _ in range(n)]
+    for s in range(n):
+        for e in range(int(index[s]), int(index[s + 1])):
+            ins[int(nbrs[e])].append(s)
+    factor = f32(1.0)
+    nnn = n
+    while nnn > 1:
+        factor = f32(float(factor) / 10.0)
+        nnn //= 10
+    dampen = f32(0.85)
+    ninv = f32(1.0) / f32(n)
+    cst = (f32(1.0) - dampen) * ninv
+    e2 = f32(eps2)
+    x = [f32(0.0)] * n
+    delta = [f32(1.0 / n)] * n
+    act = [True] * n
+    fv, fe, it, kinds = n, m, 0, []
+    for fmt, eps in stages:
+        while it < max_iter:
+            if it == 0:
+                kind = 0
+            elif (fv + fe) / m > 0.6:
+                kind = 1
+            else:
+                kind = 3 if float(f32(fv + fe)) <= float(f32(m)) / 100.0 else 2
+            allsrc = kind <= 1
+            c = {}
+            for v in range(n):
+                if (allsrc or act[v]) and deg[v]:
+                    c[v] = _enc(fmt, delta[v] * (f32(1.0) / (factor * f32(deg[v]))))
+            z = []
+            for d in range(n):
+                if fmt == 0:
+                    acc = 0.0
+                    for s in sorted(ins[d]):
+                        if allsrc or act[s]:
+                            acc += float(_dec(fmt, c[s])) * float(factor)
+                    z.append(f32(acc))
+                else:
+                    acc = f32(0.0)
+                    for s in sorted(ins[d]):
+                        if allsrc or act[s]:
+                            acc = f32(acc + _dec(fmt, c[s]) * factor)
+                    z.append(acc)
+            l1, fv, fe, nact = 0.0, 0, 0, []
+            for v in range(n):
+                if kind == 0:
+                    dl = dampen * z[v] + cst
+                    x[v] = x[v] + dl
+                    dl = dl - ninv
+                    a = abs(dl) > e2 * x[v]
+                else:
+                    dl = z[v] * dampen
+                    a = abs(dl) > e2 * x[v]
+                    if a:
+                        x[v] = x[v] + dl
+                delta[v] = dl
+                nact.append(a)
+                l1 += float(abs(dl))
+                fv += int(a)
+                fe += deg[v] if a else 0
+            act = nact
+            kinds.append(kind)
+            it += 1
+            if l1 < eps:
+                break
+    ss = sum(abs(float(v)) for v in x)
+    return np.array([float(v) * (1.0 / ss) for v in x]), it, kinds
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("stages", [((0, 1e-4),), ((0, 1e-1), (5, 1e-6))])
+def test_apr_branches_brute(seed, stages):
+    if seed % 2:
+        index, nbrs = O.rmat_graph(6, 4, seed=seed + 1)
+        cindex, cnbrs = index, nbrs
+    else:
+        index, nbrs, cindex, cnbrs, _ = directed_graph(64, 400, seed)
+    want, wit, wkinds = apr_brute(index, nbrs, stages, 40, 0.05)
+    x, it, l1, ed, tr = O.apr_vary(index, nbrs, stages=stages, max_iter=40, eps2=0.05, outdeg=O.degrees(index),
+                                   cindex=cindex, cnbrs=cnbrs, threads=1)
+    assert it == wit
+    assert [int(k) for k in tr[:, 0]] == wkinds
+    assert np.array_equal(x, want)
+    assert {1, 2, 3} & set(wkinds)   # the frontier branches are exercised
+
+
+def test_apr_default_stages_near_pagerank():
+    index, nbrs = O.rmat_graph(14)
+    x, it, l1, ed, tr = O.apr_vary(index, nbrs)
+    assert l1 < 1e-7 and it < 100
+    kinds = set(int(k) for k in tr[:, 0])
+    assert kinds >= {0, 1, 3}
+    xp, _, _ = O.pr_fp64(index, nbrs, 200, 1e-13)
+    assert np.abs(x - xp / xp.sum()).sum() < 1e-2
+
+
+def test_apr_rejects_unsigned_formats():
+    index, nbrs = O.rmat_graph(6)
+    for f in (2, 3):
+        with pytest.raises(ValueError):
+            O.apr_vary(index, nbrs, stages=((f, 1e-3),))
