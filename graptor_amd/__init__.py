@@ -117,6 +117,7 @@ SIGNATURES = {
     "gpr_ccv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_prv": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_pr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
+    "gpr_kcore": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(C.c_uint32), C.POINTER(RunInfo)]),
     "gpr_apr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.c_double, C.POINTER(RunOpts), _P,
                                C.POINTER(RunInfo)]),
     "gpr_msbfs": (C.c_int, [_P, _P, C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
@@ -344,6 +345,16 @@ class Graph:
         _check(lib().gpr_apr_vary(self.h, arr, len(stages), eps2, C.byref(o), _ptr(out) if download else None,
                                   C.byref(info)))
         return out, info
+
+    def kcore(self, download=True, **kw):
+        """Core numbers (KC_bucket's coreness); returns (coreness, largest
+        core, RunInfo)."""
+        out = np.empty(self.n, np.uint32) if download else None
+        o = opts(**kw)
+        info = RunInfo()
+        kmax = C.c_uint32()
+        _check(lib().gpr_kcore(self.h, C.byref(o), _ptr(out) if download else None, C.byref(kmax), C.byref(info)))
+        return out, kmax.value, info
 
     def msbfs(self, sources, **kw):
         """Levels from up to 64 sources in one traversal, shape (k, n)."""

@@ -7,6 +7,7 @@
 //   PRv       bench/PRv.C:115-290      (iteration :201-280)
 //   PR_vary   bench/PR_vary.C:247-757  (run_prec :247-447, staged run :652-757)
 //   APR_vary  bench/APR_vary.C:378-1074 (run_prec :378-640, run :900-1074)
+//   KC_bucket bench/KC_bucket.C:104-330  (run; validate :332-460)
 //
 // Each benchmark runs the reference's DEFERRED_UPDATE=0 configuration of
 // the record operator (api::record(output, api::reduction, api::strong),
@@ -15,6 +16,7 @@
 // same vertices as the changed-vs-previous method and needs no prev copy.
 #include "../../include/gpr/api.hpp"
 #include "apr.hpp"
+#include "kcore.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -443,7 +445,44 @@ struct APRv {
     }
 };
 
+// ---- KC_bucket -----------------------------------------------------------------
+// Coreness by peeling (the count-down relax of KC_bucket.C:251-281 applied
+// level by level); the output is the core number of every vertex.
+struct KCv {
+    Graph GA;
+    gpr_run_opts o;
+    A::vertexprop<uint32_t> coreness;
+    uint32_t largestCore = 0;
+    KCv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), coreness(GA, "count-down degrees / coreness") {}
+    ~KCv() { coreness.del(); }
+
+    void run(gpr_run_info *info) {
+        KcStats st = {};
+        check(kcore_peel(GA.get(), coreness.get(), &st), "kcore");
+        largestCore = st.kmax;
+        info->iters = (int)st.levels;
+        info->push_iters = (int)st.rounds;
+        info->edges = st.edges;
+    }
+};
+
 } // namespace
+
+extern "C" int gpr_kcore(gpr_graph *g, const gpr_run_opts *opts, uint32_t *coreness, uint32_t *largest,
+                         gpr_run_info *info) {
+    if (!g) return GPR_E_INVAL;
+    gpr_run_opts o = defaults(opts);
+    return guarded([&] {
+        KCv app(g, o);
+        gpr_run_info local = {};
+        stream_time_ms(g, true);
+        app.run(&local);
+        local.ms = stream_time_ms(g, false);
+        if (coreness) app.coreness.download(coreness);
+        if (largest) *largest = app.largestCore;
+        if (info) *info = local;
+    });
+}
 
 extern "C" int gpr_apr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages, double eps2,
                             const gpr_run_opts *opts, double *x, gpr_run_info *info) {

@@ -48,6 +48,7 @@
 // leaves when the flag is set, so the grid always drains.
 #include "internal.hpp"
 #include "device.hpp"
+#include "grid_sync.hpp"
 
 namespace gpr {
 
@@ -57,14 +58,9 @@ constexpr uint32_t FU_LIGHT_MAX = 32;   // rows up to this length: lane per vert
 constexpr uint32_t FU_SLICE = 256;      // longer rows: one wave per slice of this many edges
 constexpr int FU_U = 4;                 // edges per lane in flight (FU_SLICE = 64 * FU_U)
 static_assert(FU_SLICE == 64 * FU_U, "a slice is one pass of the wave");
-// s_waitcnt vmcnt(0) (expcnt and lgkmcnt left at their maxima): waits until
-// this wave's earlier memory operations, stores and atomics included, are
-// acknowledged
-constexpr int WAIT_VM0 = 0x0f70;
-
-constexpr int FU_GROUP = 16;            // blocks per first-level barrier counter
-constexpr int FU_MAX_GROUPS = 64;
-constexpr int FU_LINE = 16;             // u64 per 128-byte line: one counter per line
+constexpr int FU_GROUP = GS_GROUP;
+constexpr int FU_MAX_GROUPS = GS_MAX_GROUPS;
+constexpr int FU_LINE = GS_LINE;
 
 struct FusedCtl {
     unsigned long long arrive;       // groups arrived at the barrier (monotonic)
@@ -88,15 +84,6 @@ struct FusedLists {
     uint64_t *bits[2];
 };
 
-__device__ __forceinline__ unsigned long long ld_agent(const unsigned long long *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned long long bcast64(unsigned long long v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((unsigned long long)hi << 32) | lo;
-}
 
 __device__ __forceinline__ uint32_t slices_of(uint32_t dg) {
     return dg <= FU_LIGHT_MAX ? 0u : (dg + FU_SLICE - 1) / FU_SLICE;
@@ -204,50 +191,6 @@ __device__ __forceinline__ void relax_edges(const bool (&act)[FU_U], const uint3
     if (__ballot(any)) append_next(push, d, dg, L, nb, nlight, nheavy, nedge);
 }
 
-// Grid barrier in two levels.  Block b arrives on the counter of group
-// b % ngroups (at most FU_GROUP blocks each); the last block of a group to
-// arrive at level `lvl` counts the group on `arrive`, and level `lvl` ends
-// when (lvl+1)*ngroups groups are counted.  All counters are monotonic.
-// Returns false (in every thread) when the error flag is set or the wait saw
-// no new arrival for `spin_limit` polls in a row.
-__device__ __forceinline__ bool grid_barrier(FusedCtl *ctl, unsigned lvl, unsigned spin_limit, int *s_ok) {
-    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned ngroups = (gridDim.x + FU_GROUP - 1) / FU_GROUP;
-        const unsigned grp = blockIdx.x % ngroups;
-        const unsigned gsize = gridDim.x / ngroups + (grp < gridDim.x % ngroups ? 1u : 0u);
-        const unsigned long long got = __hip_atomic_fetch_add(&ctl->group[grp * FU_LINE], 1ull, __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT);
-        if (got + 1 == (unsigned long long)(lvl + 1) * gsize)
-            __hip_atomic_fetch_add(&ctl->arrive, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long target = (unsigned long long)(lvl + 1) * ngroups;
-        unsigned long long seen = ~0ull;
-        unsigned spin = 0;
-        int ok = 1;
-        for (;;) {
-            const unsigned long long a = ld_agent(&ctl->arrive);
-            if (a >= target) break;
-            if (__hip_atomic_load(&ctl->error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                ok = 0;
-                break;
-            }
-            if (a != seen) {
-                seen = a;
-                spin = 0;
-            } else if (++spin > spin_limit) {
-                __hip_atomic_fetch_or(&ctl->error, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        *s_ok = ok;
-    }
-    __syncthreads();
-    return *s_ok != 0;
-}
-
 template <int RELAX>
 __global__ void __launch_bounds__(FU_THREADS)
 k_fused(const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs, const uint32_t *__restrict__ deg,
@@ -340,7 +283,7 @@ k_fused(const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs, c
             }
             relax_edges<RELAX>(act, d, val, prop, deg, L, nb, nlight, nheavy, nedge);
         }
-        if (!grid_barrier(ctl, lvl, spin_limit, &s_ok)) break;
+        if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, lvl, spin_limit, &s_ok)) break;
     }
     edges = wave_sum(edges);
     pops = wave_sum(pops);

@@ -137,6 +137,8 @@ static void graph_free(gpr_graph *g) {
     cold_free(g);
     dev_free(g->fbuf);
     dev_free(g->fctl);
+    dev_free(g->kbuf);
+    dev_free(g->kctl);
     if (g->ev0) hipEventDestroy(g->ev0);
     if (g->ev1) hipEventDestroy(g->ev1);
     if (g->rev0) hipEventDestroy(g->rev0);

@@ -198,6 +198,9 @@ struct gpr_graph {
     void *fbuf = nullptr;              // fused push: level lists and second bitmap (built on first use)
     size_t fbuf_bytes = 0;
     void *fctl = nullptr;              // fused push: control block
+    void *kbuf = nullptr;              // k-core peeling: candidate and frontier lists (built on first use)
+    size_t kbuf_bytes = 0;
+    void *kctl = nullptr;              // k-core peeling: control block
 };
 
 struct gpr_prop {

@@ -302,6 +302,15 @@ int gpr_pr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages,
 int gpr_apr_vary(gpr_graph *g, const gpr_pr_stage *stages, int nstages, double eps2,
                  const gpr_run_opts *o, double *x, gpr_run_info *info);
 
+/* KC_bucket (bench/KC_bucket.C:104-330): coreness[n] = the core number of
+ * every vertex (original IDs), *largest = the largest (largestCore).  The
+ * graph must be symmetric.  info->iters = distinct core values peeled,
+ * info->push_iters = count-down rounds, info->edges = out-edges of peeled
+ * vertices.  The reference's -kc:buckets (open buckets) has no counterpart:
+ * the peeling takes the next core value from a device-wide minimum. */
+int gpr_kcore(gpr_graph *g, const gpr_run_opts *o, uint32_t *coreness, uint32_t *largest,
+              gpr_run_info *info);
+
 /* Multi-source BFSLVLv: k <= 64 sources (original IDs) as the bit lanes of
  * one traversal; levels[j*n + v] is the level of v from sources[j] (n+1
  * unreached), each lane equal to gpr_bfslvl from that source.  No

@@ -73,6 +73,8 @@ def lib():
         L.orc_apr_vary.argtypes = [C.c_uint32, C.c_uint64, u64p, u32p, u32p, C.POINTER(C.c_int),
                                    C.POINTER(C.c_double), C.c_int, C.c_int, C.c_double, f64p, C.c_int,
                                    C.POINTER(C.c_double), C.POINTER(C.c_uint64), C.c_void_p]
+        L.orc_kcore.restype = C.c_uint32
+        L.orc_kcore.argtypes = [C.c_uint32, u64p, u32p, u32p]
         L.orc_cfp16_encode.restype = C.c_uint16
         L.orc_cfp16_encode.argtypes = [C.c_float]
         L.orc_cfp16_decode.restype = C.c_float
@@ -324,6 +326,14 @@ def apr_vary(index, nbrs, stages=APR_STAGES_2046, max_iter=100, eps2=1e-2, threa
     if it < 0:
         raise ValueError("apr_vary: contribution format without a sign")
     return x[:n], it, l1.value, ed.value, tr[:it]
+
+
+def kcore(index, nbrs):
+    """Core numbers (KC_bucket's coreness) and the largest one."""
+    n = len(index) - 1
+    out = np.empty(max(n, 1), np.uint32)
+    kmax = lib().orc_kcore(n, np.ascontiguousarray(index, np.uint64), np.ascontiguousarray(nbrs, np.uint32), out)
+    return out[:n], int(kmax)
 
 
 def pr_dense_fp64(index, nbrs, iters, damping=0.85):

@@ -745,3 +745,59 @@ int orc_apr_vary(uint32_t n, uint64_t m, const uint64_t *cindex, const uint32_t 
     if (edges_out) *edges_out = edges;
     return iter;
 }
+
+/* k-core numbers (KC_bucket, bench/KC_bucket.C:104-330): the output of the
+ * reference's bucketed peeling is the core number of every vertex, which is
+ * unique; it is computed here by the textbook O(n+m) bucket algorithm
+ * (Batagelj and Zaversnik, 2003): vertices sorted by current degree, each
+ * taken in order lowers the degree of its higher-degree neighbours by one.
+ * Symmetric graphs only.  Returns the largest core number (largestCore,
+ * KC_bucket.C:228). */
+uint32_t orc_kcore(uint32_t n, const uint64_t *index, const uint32_t *nbrs, uint32_t *core) {
+    if (n == 0) return 0;
+    uint32_t md = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        core[v] = (uint32_t)(index[v + 1] - index[v]);
+        if (core[v] > md) md = core[v];
+    }
+    uint32_t *bin = (uint32_t *)calloc((size_t)md + 2, sizeof(uint32_t));
+    uint32_t *pos = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    uint32_t *vert = (uint32_t *)malloc(sizeof(uint32_t) * n);
+    for (uint32_t v = 0; v < n; ++v) bin[core[v]]++;
+    uint32_t start = 0;
+    for (uint32_t d = 0; d <= md; ++d) {
+        uint32_t num = bin[d];
+        bin[d] = start;
+        start += num;
+    }
+    for (uint32_t v = 0; v < n; ++v) {
+        pos[v] = bin[core[v]];
+        vert[pos[v]] = v;
+        bin[core[v]]++;
+    }
+    for (uint32_t d = md; d >= 1; --d) bin[d] = bin[d - 1];
+    bin[0] = 0;
+    uint32_t kmax = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t v = vert[i];
+        if (core[v] > kmax) kmax = core[v];
+        for (uint64_t e = index[v]; e < index[v + 1]; ++e) {
+            const uint32_t u = nbrs[e];
+            if (core[u] > core[v]) {
+                const uint32_t du = core[u], pu = pos[u], pw = bin[du], w = vert[pw];
+                if (u != w) {
+                    pos[u] = pw;
+                    vert[pu] = w;
+                    pos[w] = pu;
+                    vert[pw] = u;
+                }
+                bin[du]++;
+                core[u]--;
+            }
+        }
+    }
+    free(bin);
+    free(pos);
+    free(vert);
+    return kmax;
+}

@@ -129,6 +129,10 @@ int orc_apr_vary(uint32_t n, uint64_t m, const uint64_t *cindex, const uint32_t 
                  int max_iter, double eps2, double *x_out, int threads, double *l1_out,
                  uint64_t *edges_out, double *trace);
 
+/* Core numbers of a symmetric graph (bench/KC_bucket.C:104-330 computes
+ * them by bucketed peeling); returns the largest. */
+uint32_t orc_kcore(uint32_t n, const uint64_t *index, const uint32_t *nbrs, uint32_t *core);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,6 +23,10 @@ each oracle function is pinned by mathematics instead:
   orc_scfp16_*         exhaustive 2^16 decode against the closed form
                        (-1)^s (1 + m/512) 2^(e-63) (code 0 -> 0), truncation
                        toward zero of every encode, the exponent aliasing
+  orc_kcore            brute force from the definition (core number = the
+                       largest k whose k-core, left by deleting vertices of
+                       degree < k until none remains, holds the vertex) on
+                       random, RMAT, complete and path graphs
   orc_apr_vary         with eps2 = 0 the delta iteration equals the FP64
                        dense power iteration p' = d*P*p + (1-d)/n after the
                        same number of iterations (directed graphs with sinks
@@ -556,3 +560,57 @@ def test_apr_rejects_unsigned_formats():
     for f in (2, 3):
         with pytest.raises(ValueError):
             O.apr_vary(index, nbrs, stages=((f, 1e-3),))
+
+
+# ---- k-core (KC_bucket) ---------------------------------------------------------
+def brute_core(index, nbrs):
+    n = len(index) - 1
+    adj = [set(int(u) for u in nbrs[index[v]:index[v + 1]]) for v in range(n)]
+    core = [0] * n
+    k = 1
+    alive = set(range(n))
+    while alive:
+        # the k-core: delete vertices of degree < k until none is left
+        changed = True
+        while changed:
+            changed = False
+            for v in list(alive):
+                if len(adj[v] & alive) < k:
+                    alive.discard(v)
+                    changed = True
+        for v in alive:
+            core[v] = k
+        k += 1
+    return np.array(core, np.uint32)
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_kcore_brute(case):
+    rng = np.random.default_rng(100 + case)
+    if case < 4:
+        n = int(rng.integers(5, 60))
+        cnt = int(rng.integers(0, 4 * n))
+        index, nbrs = O.build_csr(n, rng.integers(0, n, cnt).astype(np.uint32),
+                                  rng.integers(0, n, cnt).astype(np.uint32))
+    elif case < 6:
+        index, nbrs = O.rmat_graph(7 + case - 4, 8, seed=case)
+    elif case == 6:   # complete graph K_12 plus a path hanging off it
+        src = [i for i in range(12) for j in range(12) if i < j] + list(range(11, 20))
+        dst = [j for i in range(12) for j in range(12) if i < j] + list(range(12, 21))
+        index, nbrs = O.build_csr(21, np.array(src, np.uint32), np.array(dst, np.uint32))
+    else:             # isolated vertices only
+        index, nbrs = np.zeros(6, np.uint64), np.zeros(0, np.uint32)
+    want = brute_core(index, nbrs)
+    got, kmax = O.kcore(index, nbrs)
+    assert np.array_equal(got, want)
+    assert kmax == (int(want.max()) if len(want) else 0)
+    if case == 6:
+        assert (got[:11] == 11).all() and (got[12:] == 1).all()
+
+
+def test_kcore_planted_mistake_fails_brute():
+    index, nbrs = O.rmat_graph(8, 8)
+    got, _ = O.kcore(index, nbrs)
+    wrong = got.copy()
+    wrong[np.argmax(wrong)] -= 1
+    assert not np.array_equal(wrong, brute_core(index, nbrs))
