@@ -13,10 +13,13 @@
 // Here the whole peeling is ONE cooperative launch (the levels are many and
 // mostly small: hundreds of distinct core values and several rounds each
 // on RMAT graphs), with grid barriers between its phases:
-//   level K, pass A  minimum count over the candidate list (vertices not yet
-//                    peeled; level 0 takes every vertex without a list);
+//   level K, pass A  minimum count over the candidates (vertices not yet
+//                    peeled): each wave owns one segment of them for the
+//                    whole launch (level 0: a range of vertex IDs);
 //   level K, pass B  candidates with count K form the frontier, those above
-//                    K the next level's candidate list (peeled ones drop);
+//                    K stay in the segment, compacted in place (the count
+//                    lives in the wave's registers: no atomics), peeled ones
+//                    drop;
 //   round            the frontier's out-edges count their destinations down;
 //                    the one decrement that takes a destination from K+1 to
 //                    K appends it to the next round's frontier, so every
@@ -27,10 +30,12 @@
 // decrement at the floor) is undone with atomicAdd before the round's
 // barrier, so after every barrier each count is exact.
 // Frontier lists as in the fused push: rows up to KC_LIGHT_MAX edges one
-// lane per vertex, longer rows one wave per KC_SLICE-edge slice; appends are
-// wave-aggregated.  List counts rotate over three slots per round, candidate
-// counts and level minima over two slots per level, each cleared one phase
-// after its last read and at least one barrier before its next write.
+// lane per vertex, longer rows one wave per KC_SLICE-edge slice.  Light
+// appends are staged per wave in LDS and reserved in the shared list once
+// per KC_STAGE entries, heavy slices are reserved per wave step.  List
+// counts rotate over three slots per round and level minima over two slots
+// per level, each cleared one phase after its last read and at least one
+// barrier before its next write.
 // Memory ordering and the barrier: grid_sync.hpp.  Termination: every block
 // reads the same minimum after a barrier and leaves when no candidate
 // remains; a barrier that makes no progress for the spin limit sets the
@@ -50,26 +55,34 @@ constexpr int KC_WAVES = KC_THREADS / 64;
 constexpr uint32_t KC_LIGHT_MAX = 32;
 constexpr uint32_t KC_SLICE = 256;
 constexpr int KC_U = 4;
+constexpr int KC_STAGE = 512;           // light appends staged per wave in LDS
+constexpr int KC_PU = 4;                // candidate chunks per wave step in passes A and B
+constexpr uint32_t KC_HUBS = 8192;      // count-downs of vertices below this ID add up in LDS
+static_assert(KC_HUBS % KC_THREADS == 0, "whole passes over the hub counters");
+constexpr unsigned KC_MAX_WAVES = GS_GROUP * GS_MAX_GROUPS * KC_WAVES;
 static_assert(KC_SLICE == 64 * KC_U, "a slice is one pass of the wave");
+// phase kinds timed by block 0: pass A, its barrier, pass B, its barrier,
+// round work, round barriers
+constexpr int KC_PROF = 6;
 
 struct KcCtl {
     unsigned long long arrive;        // groups arrived at the barrier (monotonic)
     unsigned long long nlight[3];     // frontier list lengths, rotating per round
     unsigned long long nheavy[3];
-    unsigned long long ncand[2];      // candidate list lengths, alternating per level
     unsigned long long edges;         // out-edges of peeled vertices examined
     unsigned int error;               // 1: a barrier wait made no progress
     unsigned int levels;              // levels with a non-empty frontier
     unsigned int rounds;              // rounds run
     unsigned int kmax;                // largest core number
     unsigned int kmin[2];             // level minima, alternating
-    unsigned long long pad[GS_LINE - 13];
+    unsigned long long pad[GS_LINE - 11];
     unsigned long long group[GS_MAX_GROUPS * GS_LINE];
+    unsigned long long prof[KC_PROF];   // block 0's wall-clock ticks per phase kind
 };
 static_assert(offsetof(KcCtl, group) == GS_LINE * sizeof(unsigned long long), "one line of counters");
 
 struct KcLists {
-    uint32_t *cand[2];
+    uint32_t *cand;                   // per-wave segments of the candidates
     uint32_t *light[2];
     uint64_t *heavy[2];               // (vertex << 32) | slice
 };
@@ -85,11 +98,34 @@ __device__ __forceinline__ uint32_t kc_slices(uint32_t dg) {
     return dg <= KC_LIGHT_MAX ? 0u : (dg + KC_SLICE - 1) / KC_SLICE;
 }
 
-// Append the vertices v[j] with put[j] to the frontier lists of slot c3 /
-// buffer cb (light or heavy by degree).  Wave-uniform: every lane calls it.
+// Light appends of one wave are staged in its own LDS buffer and written
+// out with one reservation per KC_STAGE entries at most, so the shared list
+// counter sees one atomic per buffer instead of one per wave step.  The
+// stage's count is wave-uniform; each phase flushes before its barrier.
+struct KcStage {
+    uint32_t *buf;   // this wave's KC_STAGE entries of LDS
+    uint32_t n;      // entries staged
+};
+
+__device__ __forceinline__ void kc_flush(KcStage &st, uint32_t *list, unsigned long long *nlight) {
+    if (st.n == 0) return;
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long b = 0;
+    if (lane() == 0)
+        b = __hip_atomic_fetch_add(nlight, (unsigned long long)st.n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    b = bcast64(b);
+    for (uint32_t i = lane(); i < st.n; i += 64)
+        __hip_atomic_store(&list[b + i], st.buf[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_wave_barrier();
+    st.n = 0;
+}
+
+// Append the vertices v[j] with put[j] to the frontier lists of buffer cb:
+// light rows through the stage, heavy rows' slices directly.  Wave-uniform:
+// every lane calls it.
 __device__ __forceinline__ void kc_append(const bool (&put)[KC_U], const uint32_t (&v)[KC_U],
                                           const uint32_t *__restrict__ deg, const KcLists &L, int cb,
-                                          unsigned long long *nlight, unsigned long long *nheavy) {
+                                          unsigned long long *nlight, unsigned long long *nheavy, KcStage &st) {
     uint32_t dg[KC_U];
     uint32_t cl = 0, ch = 0;
 #pragma unroll
@@ -104,20 +140,24 @@ __device__ __forceinline__ void kc_append(const bool (&put)[KC_U], const uint32_
     if (!__ballot(cl | ch)) return;
     const uint32_t il = wave_incl_scan(cl), ih = wave_incl_scan(ch);
     const uint32_t tl = __shfl(il, 63, 64), th = __shfl(ih, 63, 64);
-    unsigned long long bl = 0, bh = 0;
-    if (lane() == 0) {
-        if (tl) bl = __hip_atomic_fetch_add(nlight, (unsigned long long)tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (th) bh = __hip_atomic_fetch_add(nheavy, (unsigned long long)th, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    bl = bcast64(bl) + (il - cl);
-    bh = bcast64(bh) + (ih - ch);
+    if (tl) {
+        if (st.n + tl > (uint32_t)KC_STAGE) kc_flush(st, L.light[cb], nlight);
+        uint32_t p = st.n + (il - cl);
 #pragma unroll
-    for (int j = 0; j < KC_U; ++j) {
-        if (put[j]) {
-            const uint32_t s = kc_slices(dg[j]);
-            if (s == 0) {
-                __hip_atomic_store(&L.light[cb][bl++], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            } else {
+        for (int j = 0; j < KC_U; ++j)
+            if (put[j] && kc_slices(dg[j]) == 0) st.buf[p++] = v[j];
+        st.n += tl;
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (th) {
+        unsigned long long bh = 0;
+        if (lane() == 0)
+            bh = __hip_atomic_fetch_add(nheavy, (unsigned long long)th, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bh = bcast64(bh) + (ih - ch);
+#pragma unroll
+        for (int j = 0; j < KC_U; ++j) {
+            if (put[j]) {
+                const uint32_t s = kc_slices(dg[j]);
                 for (uint32_t k = 0; k < s; ++k)
                     __hip_atomic_store(&L.heavy[cb][bh++], ((uint64_t)v[j] << 32) | k, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
@@ -130,10 +170,18 @@ __device__ __forceinline__ void kc_append(const bool (&put)[KC_U], const uint32_
 // those that reach K go to the next frontier.  Wave-uniform.
 __device__ __forceinline__ void kc_count_down(const bool (&act)[KC_U], const uint32_t (&d)[KC_U], uint32_t K,
                                               uint32_t *core, const uint32_t *__restrict__ deg, const KcLists &L,
-                                              int nb, unsigned long long *nlight, unsigned long long *nheavy) {
+                                              int nb, unsigned long long *nlight, unsigned long long *nheavy,
+                                              KcStage &st, uint32_t *hub) {
     bool need[KC_U];
 #pragma unroll
     for (int j = 0; j < KC_U; ++j) need[j] = act[j] && core[d[j]] > K;
+    // hubs: counted in this block's LDS, applied once per round (kc_hub_flush)
+#pragma unroll
+    for (int j = 0; j < KC_U; ++j)
+        if (need[j] && d[j] < KC_HUBS) {
+            atomicAdd(&hub[d[j]], 1u);
+            need[j] = false;
+        }
     uint32_t old[KC_U];
 #pragma unroll
     for (int j = 0; j < KC_U; ++j) old[j] = need[j] ? atomicSub(&core[d[j]], 1u) : 0u;
@@ -143,75 +191,143 @@ __device__ __forceinline__ void kc_count_down(const bool (&act)[KC_U], const uin
         if (need[j] && old[j] <= K) atomicAdd(&core[d[j]], 1u);   // raced at the floor: undo
         put[j] = need[j] && old[j] == K + 1;
     }
-    kc_append(put, d, deg, L, nb, nlight, nheavy);
+    kc_append(put, d, deg, L, nb, nlight, nheavy, st);
+}
+
+// Apply this block's hub count-downs of the round: subtract each sum delta
+// at once.  The count falls to max(K, c - delta) as delta single steps would:
+// a subtraction that starts at or below K is undone, one that crosses K is
+// topped back up to K, and that crossing one (there is exactly one per hub
+// and level) appends the hub.  Block-uniform: every thread calls it.
+__device__ __forceinline__ void kc_hub_flush(uint32_t *hub, uint32_t n, uint32_t K, uint32_t *core,
+                                             const uint32_t *__restrict__ deg, const KcLists &L, int nb,
+                                             unsigned long long *nlight, unsigned long long *nheavy, KcStage &st) {
+    __syncthreads();
+    for (uint32_t base = 0; base < KC_HUBS; base += KC_THREADS) {
+        const uint32_t j = base + threadIdx.x;
+        const uint32_t dl = hub[j];
+        bool put[KC_U] = {false, false, false, false};
+        uint32_t v[KC_U] = {j, j, j, j};
+        if (dl) {
+            hub[j] = 0;
+            const uint32_t old = atomicSub(&core[j], dl);
+            if (old <= K) {
+                atomicAdd(&core[j], dl);
+            } else if (old - dl <= K) {
+                if (old - dl < K) atomicAdd(&core[j], K - (old - dl));
+                put[0] = true;
+            }
+        }
+        kc_append(put, v, deg, L, nb, nlight, nheavy, st);
+    }
 }
 
 __global__ void __launch_bounds__(KC_THREADS)
 k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs,
         const uint32_t *__restrict__ deg, uint32_t *core, KcLists L, KcCtl *ctl, unsigned spin_limit) {
     __shared__ int s_ok;
+    __shared__ uint32_t s_stage[KC_WAVES * KC_STAGE];
+    __shared__ uint32_t s_hub[KC_HUBS];
+    for (uint32_t j = threadIdx.x; j < KC_HUBS; j += KC_THREADS) s_hub[j] = 0;
+    __syncthreads();
     const unsigned ln = lane();
     const unsigned long long W = (unsigned long long)gridDim.x * KC_WAVES;
     // wave rank interleaved over the blocks: small phases land on many CUs
     const unsigned long long wid = (unsigned long long)(threadIdx.x / 64) * gridDim.x + blockIdx.x;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
+    KcStage st = {&s_stage[(threadIdx.x / 64) * KC_STAGE], 0u};
+    // this wave's segment of the candidates: level 0 takes the vertices
+    // [wid*S, wid*S + S) without a list; each pass B compacts the survivors
+    // in place, so the count stays in a register and needs no atomic
+    const unsigned long long S = ((unsigned long long)n + W - 1) / W;
+    uint32_t *seg = L.cand + wid * S;
+    uint32_t cnt = (uint32_t)(wid * S >= n ? 0ull : ((unsigned long long)n - wid * S < S ? n - wid * S : S));
     unsigned ph = 0;          // barriers passed
     unsigned r = 0;           // rounds run
     uint32_t lo = 0;          // vertices not yet peeled have a count >= lo
     unsigned long long my_edges = 0;
+    // block 0's phase times live in LDS (registers are the scarce resource)
+    __shared__ unsigned long long s_prof[KC_PROF + 1];
+    if (leader) {
+        for (int k = 0; k < KC_PROF; ++k) s_prof[k] = 0;
+        s_prof[KC_PROF] = wall_clock64();
+    }
+    auto mark = [&](int k) {
+        if (leader) {
+            const unsigned long long t = wall_clock64();
+            s_prof[k] += t - s_prof[KC_PROF];
+            s_prof[KC_PROF] = t;
+        }
+    };
     bool ok = true;
     for (unsigned lvl = 0; ok; ++lvl) {
         const int lb = lvl & 1, nlb = (lvl + 1) & 1;
-        const unsigned long long nc = lvl == 0 ? n : ld_agent(&ctl->ncand[lb]);
-        const uint32_t *cand = L.cand[lb];
         if (leader) {
-            // last read in level lvl-1 (before two barriers ago); written
-            // again after this level's first barrier
-            __hip_atomic_store(&ctl->ncand[nlb], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // last read in level lvl-1, next written after this level's
+            // first barrier
             __hip_atomic_store(&ctl->kmin[nlb], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // pass A: the smallest count among the candidates not yet peeled
+        // pass A: the smallest count among this wave's candidates
         uint32_t mn = ~0u;
-        for (unsigned long long i = wid * 64 + ln; i < nc; i += W * 64) {
-            const uint32_t v = lvl == 0 ? (uint32_t)i : ld32(&cand[i]);
-            const uint32_t c = ld32(&core[v]);
-            if (c >= lo && c < mn) mn = c;
+        for (uint32_t base = 0; base < cnt; base += 64 * KC_PU) {
+            uint32_t v[KC_U] = {0u, 0u, 0u, 0u}, c[KC_U];
+#pragma unroll
+            for (int u = 0; u < KC_PU; ++u) {
+                const uint32_t i = base + u * 64 + ln;
+                v[u] = i >= cnt ? 0u : lvl == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
+            }
+#pragma unroll
+            for (int u = 0; u < KC_PU; ++u) c[u] = base + u * 64 + ln < cnt ? ld32(&core[v[u]]) : ~0u;
+#pragma unroll
+            for (int u = 0; u < KC_PU; ++u)
+                if (c[u] >= lo && c[u] < mn) mn = c[u];
         }
         mn = wave_min(mn);
         if (ln == 0 && mn != ~0u)
             __hip_atomic_fetch_min(&ctl->kmin[lb], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mark(0);
         if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) break;
+        mark(1);
         const uint32_t K = __hip_atomic_load(&ctl->kmin[lb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (K == ~0u) break;                       // every vertex peeled
-        // pass B: count K -> this level's frontier; above K -> candidates
+        // pass B: count K -> this level's frontier; above K -> kept in the
+        // segment (written at or below the position read, so in place)
         {
             const int c3 = r % 3, cb = r & 1;
-            for (unsigned long long base = wid * 64; base < nc; base += W * 64) {
-                const unsigned long long i = base + ln;
-                uint32_t v = 0, c = 0;
-                if (i < nc) {
-                    v = lvl == 0 ? (uint32_t)i : ld32(&cand[i]);
-                    c = ld32(&core[v]);
+            uint32_t p = 0;
+            for (uint32_t base = 0; base < cnt; base += 64 * KC_PU) {
+                uint32_t v[KC_U] = {0u, 0u, 0u, 0u}, c[KC_U];
+#pragma unroll
+                for (int u = 0; u < KC_PU; ++u) {
+                    const uint32_t i = base + u * 64 + ln;
+                    v[u] = i >= cnt ? 0u : lvl == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
                 }
-                const bool live = i < nc && c >= lo;
-                const bool front = live && c == K, keep = live && c > K;
-                bool put[KC_U] = {front, false, false, false};
-                uint32_t vv[KC_U] = {v, v, v, v};
-                kc_append(put, vv, deg, L, cb, &ctl->nlight[c3], &ctl->nheavy[c3]);
-                const uint64_t km = __ballot(keep);
-                if (km) {
-                    unsigned long long b = 0;
-                    if (ln == 0)
-                        b = __hip_atomic_fetch_add(&ctl->ncand[nlb], (unsigned long long)__popcll(km),
-                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    b = bcast64(b);
-                    if (keep)
-                        __hip_atomic_store(&L.cand[nlb][b + mbcnt(km)], v, __ATOMIC_RELAXED,
-                                           __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int u = 0; u < KC_PU; ++u) c[u] = base + u * 64 + ln < cnt ? ld32(&core[v[u]]) : 0u;
+                bool put[KC_U] = {false, false, false, false}, keep[KC_U];
+#pragma unroll
+                for (int u = 0; u < KC_PU; ++u) {
+                    const bool live = base + u * 64 + ln < cnt && c[u] >= lo;
+                    put[u] = live && c[u] == K;
+                    keep[u] = live && c[u] > K;
+                }
+                kc_append(put, v, deg, L, cb, &ctl->nlight[c3], &ctl->nheavy[c3], st);
+                // every read of these KC_PU chunks precedes the writes, which
+                // land at or below the chunks' own positions
+#pragma unroll
+                for (int u = 0; u < KC_PU; ++u) {
+                    const uint64_t km = __ballot(keep[u]);
+                    if (keep[u])
+                        __hip_atomic_store(&seg[p + mbcnt(km)], v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    p += (uint32_t)__popcll(km);
                 }
             }
+            cnt = p;
+            kc_flush(st, L.light[cb], &ctl->nlight[c3]);
         }
+        mark(2);
         if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) break;
+        mark(3);
         // rounds of level K
         for (;;) {
             const int c3 = r % 3, cb = r & 1;
@@ -244,7 +360,7 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                         d[j] = act[j] ? nbrs[e + j] : 0u;
                     }
                     e = e + KC_U < e1 ? e + KC_U : e1;
-                    kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3]);
+                    kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub);
                 }
             }
             // heavy rows: one wave per slice
@@ -263,13 +379,17 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                     act[u] = e < ee;
                     d[u] = act[u] ? nbrs[e] : 0u;
                 }
-                kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3]);
+                kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub);
             }
+            kc_hub_flush(s_hub, n, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st);
+            kc_flush(st, L.light[nb], &ctl->nlight[n3]);
             ++r;
+            mark(4);
             if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) {
                 ok = false;
                 break;
             }
+            mark(5);
         }
         lo = K + 1;
         if (leader) {
@@ -278,6 +398,9 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
             __hip_atomic_store(&ctl->rounds, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    if (leader)
+        for (int k = 0; k < KC_PROF; ++k)
+            __hip_atomic_store(&ctl->prof[k], s_prof[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     my_edges = wave_sum(my_edges);
     if (ln == 0 && my_edges)
         __hip_atomic_fetch_add(&ctl->edges, my_edges, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -309,7 +432,7 @@ static KcBuf kc_layout(const gpr_graph *g) {
     b.nheavy = n + g->m / KC_SLICE + 64;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     b.cand = 0;
-    b.light = al(2 * n * sizeof(uint32_t));
+    b.light = al((n + KC_MAX_WAVES) * sizeof(uint32_t));
     b.heavy = b.light + al(2 * n * sizeof(uint32_t));
     b.total = b.heavy + 2 * b.nheavy * sizeof(uint64_t);
     return b;
@@ -346,8 +469,7 @@ int kcore_peel(gpr_graph *g, gpr_prop *core, KcStats *st) {
     char *base = (char *)g->kbuf;
     const size_t n = std::max<size_t>(g->n, 1);
     KcLists L;
-    L.cand[0] = (uint32_t *)(base + b.cand);
-    L.cand[1] = L.cand[0] + n;
+    L.cand = (uint32_t *)(base + b.cand);
     L.light[0] = (uint32_t *)(base + b.light);
     L.light[1] = L.light[0] + n;
     L.heavy[0] = (uint64_t *)(base + b.heavy);
@@ -375,6 +497,20 @@ int kcore_peel(gpr_graph *g, gpr_prop *core, KcStats *st) {
     if (h.error) {
         set_error("kcore: a grid barrier made no progress (level %u, round %u)", h.levels, h.rounds);
         return GPR_E_HIP;
+    }
+    if (g->prof.on) {
+        unsigned long long t[KC_PROF];
+        GPR_HIP(hipMemcpy(t, ctl->prof, sizeof(t), hipMemcpyDeviceToHost));
+        int dev = 0, khz = 0;
+        GPR_HIP(hipGetDevice(&dev));
+        GPR_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        static const char *names[KC_PROF] = {"kcore_pass_a", "kcore_wait_a", "kcore_pass_b",
+                                             "kcore_wait_b", "kcore_rounds", "kcore_wait_rounds"};
+        for (int k = 0; k < KC_PROF; ++k) {
+            auto &e = g->prof.acc[names[k]];
+            e.first += khz > 0 ? (double)t[k] / (double)khz : 0.0;
+            e.second += k < 4 ? h.levels : h.rounds;
+        }
     }
     st->kmax = g->n ? h.kmax : 0;
     st->levels = h.levels;
