@@ -4,10 +4,11 @@
 // driver.C:245-255), then post-process and validate each round and report
 // the average.  One binary serves the four benchmarks through -app.
 //
-//   gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary [options] <inFile>
+//   gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary|apr|kc [options] <inFile>
 //
 // Options follow the reference's names (driver.C:189-196, BFSLVLv.C:67-70,
-// CCv.C:175-178, PRv.C:73-77, PR_vary.C:197-201):
+// CCv.C:175-178, PRv.C:73-77, PR_vary.C:197-201, APR_vary.C:318-326,
+// KC_bucket.C:60-72):
 //   -b                 inFile is a b2 binary graph (the only file format)
 //   -s                 the graph is symmetric
 //   -rounds R          timed rounds (3)
@@ -15,6 +16,8 @@
 //   -pr:maxiter K      PR iteration cap (100)
 //   -pr:outfile F      write the ranks, one per line, original order
 //   -cc:outfile F      write the recoded component labels, one per line
+//   -kc:outfile F      write the coreness, one per line
+//   -kc:buckets B      accepted; the peeling has no open-bucket window
 //   -itimes            per-iteration lines on stderr (BFSLVLv.C:90-106)
 // and, without a counterpart in the reference:
 //   -rmat S            generate RMAT-S on the device instead of reading a file
@@ -25,8 +28,11 @@
 //   -fusion F          fused sparse steps: default|off|1|2 (the benchmarks'
 //                      FUSION setting, BFSLVLv.C:28-40, CCv.C:47-54)
 //   -policy F          network weights (JSON with a "params" array) for -dir policy
-//   -pr:stages L       PR_vary stages fmt:tol,... with fmt f32|bf16|f16|cfp16|bf16t
-//                      (default cfp16:1e-3,f32:1e-7, PR_vary.C:41-43 VARIANT 4620)
+//   -pr:stages L       PR_vary / APR_vary stages fmt:tol,... with fmt
+//                      f32|bf16|f16|cfp16|bf16t|scfp16 (default cfp16:1e-3,f32:1e-7
+//                      for pr_vary, PR_vary.C:41-43 VARIANT 4620; f32:1e-1,scfp16:1e-7
+//                      for apr, APR_vary.C:37-39 VARIANT 2046)
+//   -apr:eps2 E        APR_vary's activity threshold (1e-2, APR_vary.C:913)
 //   -device N          HIP device (0)
 //   -tuned F           runtime tunables file (default: tuned.json beside the
 //                      binary, when present; GPR_NO_TUNED=1 skips it)
@@ -101,8 +107,9 @@ Args parse(int argc, char **argv) {
 }
 
 void usage() {
-    std::cerr << "usage: gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary [-b] [-s] [-rounds R] [-start V]\n"
-                 "                  [-pr:maxiter K] [-pr:outfile F] [-cc:outfile F] [-itimes]\n"
+    std::cerr << "usage: gpr_driver -app bfslvl|bfsv|cc|pr|pr_vary|apr|kc [-b] [-s] [-rounds R] [-start V]\n"
+                 "                  [-pr:maxiter K] [-pr:outfile F] [-cc:outfile F] [-kc:outfile F] [-itimes]\n"
+                 "                  [-apr:eps2 E] [-kc:buckets B]\n"
                  "                  [-rmat S [-ef E] [-seed X]] [-reorder] [-dir auto|push|pull|policy]\n"
                  "                  [-threshold T] [-fusion default|off|1|2] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
                  "                  <inFile>\n";
@@ -127,6 +134,7 @@ int contrib_fmt(const std::string &s) {
     if (s == "f16") return GPR_CONTRIB_F16;
     if (s == "cfp16") return GPR_CONTRIB_CFP16;
     if (s == "bf16t") return GPR_CONTRIB_BF16T;
+    if (s == "scfp16") return GPR_CONTRIB_SCFP16;
     throw std::runtime_error("unknown contribution format " + s);
 }
 
@@ -358,6 +366,39 @@ void validate_cc(gpr_graph *g, const std::vector<uint32_t> &ids) {
         std::cerr << "Neighbour check: FAIL; #vertices: " << wrong << "\n";
 }
 
+// KC_bucket.C:332-460: no coreness above the degree, coreness 0 exactly for
+// isolated vertices, at least coreness-many neighbours of at least the same
+// coreness, at most coreness-many of a higher one.
+void validate_kc(gpr_graph *g, const std::vector<uint32_t> &core) {
+    HostGraph h(g);
+    const uint32_t n = h.n;
+    std::vector<uint32_t> deg(n);
+    for (uint32_t u = 0; u < n; ++u) deg[h.order[u]] = (uint32_t)(h.index[u + 1] - h.index[u]);
+    uint64_t above = 0, zero = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+        above += core[v] > deg[v];
+        zero += (core[v] == 0) != (deg[v] == 0);
+    }
+    uint64_t few = 0, low = 0;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(+ : few, low)
+    for (int64_t u = 0; u < (int64_t)n; ++u) {
+        const uint32_t v = h.order[u], cv = core[v];
+        uint64_t ge = 0, gt = 0;
+        for (uint64_t e = h.index[u]; e < h.index[u + 1]; ++e) {
+            const uint32_t cu = core[h.order[h.nbrs[e]]];
+            ge += cu >= cv;
+            gt += cu > cv;
+        }
+        few += ge < cv;
+        low += gt > cv;
+    }
+    if (above || zero || few || low)
+        std::cerr << "Validation failed: " << above << " above degree, " << zero << " zero/degree mismatches, "
+                  << few << " with too few neighbours, " << low << " with too low k-core number\n";
+    else
+        std::cerr << "Validation successful\n";
+}
+
 template <class T> void write_lines(const std::string &path, const std::vector<T> &v, int precision) {
     std::ofstream f(path);
     if (!f) throw std::runtime_error("cannot write " + path);
@@ -367,7 +408,8 @@ template <class T> void write_lines(const std::string &path, const std::vector<T
 
 int run(const Args &a) {
     const std::string app = a.str("-app", "");
-    if (app != "bfslvl" && app != "bfsv" && app != "cc" && app != "pr" && app != "pr_vary") {
+    if (app != "bfslvl" && app != "bfsv" && app != "cc" && app != "pr" && app != "pr_vary" && app != "apr" &&
+        app != "kc") {
         usage();
         return 2;
     }
@@ -419,16 +461,24 @@ int run(const Args &a) {
     std::cerr << "graph: n=" << n << " m=" << m << (sym ? " symmetric" : " directed") << "\n";
 
     const uint32_t start = (uint32_t)a.num("-start", 0);
-    std::vector<gpr_pr_stage> stages = app == "pr" ? std::vector<gpr_pr_stage>{{GPR_CONTRIB_F32, 1e-7}}
-                                                    : parse_stages(a.str("-pr:stages", "cfp16:1e-3,f32:1e-7"));
-    std::vector<uint32_t> out_u32(app == "bfslvl" || app == "bfsv" || app == "cc" ? n : 0);
+    std::vector<gpr_pr_stage> stages =
+        app == "pr" ? std::vector<gpr_pr_stage>{{GPR_CONTRIB_F32, 1e-7}}
+                    : parse_stages(a.str("-pr:stages", app == "apr" ? "f32:1e-1,scfp16:1e-7" : "cfp16:1e-3,f32:1e-7"));
+    const double eps2 = a.real("-apr:eps2", 1e-2);
+    const bool pr_like = app == "pr" || app == "pr_vary" || app == "apr";
+    std::vector<uint32_t> out_u32(app == "bfslvl" || app == "bfsv" || app == "cc" || app == "kc" ? n : 0);
     std::vector<float> out_f32(app == "pr" || app == "pr_vary" ? n : 0);
+    std::vector<double> out_f64(app == "apr" ? n : 0);
     double total = 0.0;
+    uint32_t kmax = 0;
     for (long r = 0; r < rounds; ++r) {
         gpr_run_info info = {};
         if (app == "bfslvl") CALL(gpr_bfslvl(g, start, &o, out_u32.data(), &info));
         else if (app == "bfsv") CALL(gpr_bfsv(g, start, &o, out_u32.data(), &info));
         else if (app == "cc") CALL(gpr_ccv(g, &o, out_u32.data(), &info));
+        else if (app == "kc") CALL(gpr_kcore(g, &o, out_u32.data(), &kmax, &info));
+        else if (app == "apr")
+            CALL(gpr_apr_vary(g, stages.data(), (int)stages.size(), eps2, &o, out_f64.data(), &info));
         else CALL(gpr_pr_vary(g, stages.data(), (int)stages.size(), &o, out_f32.data(), &info));
         const double sec = info.ms * 1e-3;
         total += sec;
@@ -436,17 +486,26 @@ int run(const Args &a) {
         std::cout << "Iterations : " << info.iters << " push: " << info.push_iters << " edges: " << info.edges
                   << " GTEPS: " << (sec > 0 ? double(info.edges) / sec * 1e-9 : 0.0) << "\n";
         // post_process
-        if (o.trace) dump_itimes(g, m, n, app == "pr" || app == "pr_vary");
-        if ((app == "pr" || app == "pr_vary") && info.iters >= o.max_iter)
-            std::cerr << "WARNING: took all allowed iterations\n";
+        if (o.trace && app != "kc") dump_itimes(g, m, n, app == "pr" || app == "pr_vary");
+        if (pr_like && info.iters >= o.max_iter) std::cerr << "WARNING: took all allowed iterations\n";
         if (app == "cc" && a.has("-cc:outfile")) write_lines(a.str("-cc:outfile", ""), out_u32, 10);
+        if (app == "kc") {
+            std::cout << "Largest core: " << kmax << "\n";
+            if (a.has("-kc:outfile")) write_lines(a.str("-kc:outfile", ""), out_u32, 10);
+        }
         if ((app == "pr" || app == "pr_vary") && a.has("-pr:outfile"))
             write_lines(a.str("-pr:outfile", ""), out_f32, std::numeric_limits<float>::max_digits10);
+        if (app == "apr" && a.has("-pr:outfile"))
+            write_lines(a.str("-pr:outfile", ""), out_f64, std::numeric_limits<double>::max_digits10);
         // validate
         if (app == "bfslvl") validate_bfs(g, start, out_u32, sym != 0);
         else if (app == "bfsv") validate_bfsv(g, start, info.iters, out_u32);
         else if (app == "cc") validate_cc(g, out_u32);
-        else {
+        else if (app == "kc") validate_kc(g, out_u32);
+        else if (app == "apr") {
+            for (uint32_t i = 0; i < 10 && i < n; ++i) std::cout << "PR[" << i << "]: " << out_f64[i] << "\n";
+            std::cout << "delta: " << info.delta << " sum: " << info.sum << "\n";
+        } else {
             for (uint32_t i = 0; i < 10 && i < n; ++i) std::cout << "PR[" << i << "]: " << out_f32[i] << "\n";
             std::cout << "delta: " << info.delta << " sum: " << info.sum << "\n";
         }

@@ -7,6 +7,8 @@ Bars: CC labels bit-exact; BFS validation lines PASS with the oracle's
 longest level and reached count; PR FP32 relative L1 <= 1e-6 vs the FP64
 oracle (20 iterations); PR_vary (cfp16 to 1e-3, then fp32 to 1e-7) L1 < 2e-6 vs the
 FP64 oracle run to 1e-7 (sum of ranks 1), with the reported delta <= 1e-7.
+APR_vary ranks within 2e-4 (L1) of the oracle's restatement (2e-6 with FP32
+contributions only); KC_bucket coreness bit-exact.
 """
 import os
 import re
@@ -140,3 +142,45 @@ def test_driver_fusion_rejects_unknown_setting(graph):
     r = subprocess.run([DRIVER, "-app", "cc", "-b", "-s", "-fusion", "3", path], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0 and "unknown -fusion 3" in (r.stdout + r.stderr)
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+def test_driver_apr_outfile(graph, tmp_path, reorder):
+    """-app apr (APR_vary, variant 2046 by default): the ranks written to
+    -pr:outfile equal the oracle's to the 16-bit-stage bar."""
+    path, index, nbrs = graph
+    outfile = str(tmp_path / "apr.txt")
+    extra = ["-reorder"] if reorder else []
+    out, _ = drive("-app", "apr", "-b", "-s", "-rounds", "2", "-pr:outfile", outfile, *extra, path)
+    running_times(out, 2)
+    got = np.loadtxt(outfile, dtype=np.float64)
+    want, it, l1, ed, tr = O.apr_vary(index, nbrs)
+    assert np.abs(got - want).sum() <= 2e-4
+    assert float(re.search(r"^delta: (\S+)", out, re.M).group(1)) < 1e-7
+    assert float(re.search(r"^PR\[0\]: (\S+)$", out, re.M).group(1)) == pytest.approx(want[0], rel=1e-3)
+
+
+def test_driver_apr_stages_and_eps2(graph, tmp_path):
+    path, index, nbrs = graph
+    outfile = str(tmp_path / "apr32.txt")
+    out, _ = drive("-app", "apr", "-b", "-s", "-rounds", "1", "-pr:stages", "f32:1e-7", "-apr:eps2", "0.05",
+                   "-pr:outfile", outfile, path)
+    got = np.loadtxt(outfile, dtype=np.float64)
+    want, *_ = O.apr_vary(index, nbrs, stages=((0, 1e-7),), eps2=0.05)
+    assert np.abs(got - want).sum() <= 2e-6
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+def test_driver_kc_outfile(graph, tmp_path, reorder):
+    """-app kc (KC_bucket): coreness written to -kc:outfile equals the
+    oracle's bit for bit; Largest core and the validation line."""
+    path, index, nbrs = graph
+    outfile = str(tmp_path / "kc.txt")
+    extra = ["-reorder"] if reorder else []
+    out, err = drive("-app", "kc", "-b", "-s", "-rounds", "2", "-kc:buckets", "127", "-kc:outfile", outfile, *extra,
+                     path)
+    running_times(out, 2)
+    want, kmax = O.kcore(index, nbrs)
+    assert np.array_equal(np.loadtxt(outfile, dtype=np.int64), want.astype(np.int64))
+    assert out.count("Largest core: %d" % kmax) == 2
+    assert err.count("Validation successful") == 2
