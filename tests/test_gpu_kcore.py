@@ -5,7 +5,9 @@ Core numbers are unique, so the GPU's coreness must equal the oracle's
 the largest core must equal largestCore.  The reference's own validation
 (KC_bucket.C:332-460) is checked as well: no vertex has a coreness above
 its degree, coreness 0 exactly for isolated vertices, and every vertex has
-at least coreness-many neighbours of at least its coreness.
+at least coreness-many neighbours of at least its coreness.  The one
+cooperative launch is also run through ~25000 barriers (a long path) and
+through its failure path, forced from the host with a zero spin limit.
 """
 import numpy as np
 import pytest
@@ -113,3 +115,28 @@ def test_kcore_rejects_directed():
     with G.Graph.from_csr(index, nbrs, symmetric=False) as g:
         with pytest.raises(G.GprError):
             g.kcore()
+
+
+def test_kcore_long_path_and_forced_barrier_failure():
+    """A path of 50000 vertices peels one vertex from each end per round:
+    about 25000 rounds (barriers) in one launch, coreness 1 everywhere.
+    With the spin limit at 0 (forced from the host) a barrier wait gives up
+    at once: the call fails with the k-core error, every wave leaves (the
+    error flag ends every wait), and the next run on the handle is exact."""
+    n = 50000
+    s = np.arange(n - 1, dtype=np.uint32)
+    index, nbrs = O.build_csr(n, s, s + 1)
+    old = G.tune_get("fusion_spin_limit")
+    try:
+        with G.Graph.from_csr(index, nbrs) as g:
+            core, kmax, info = g.kcore()
+            assert (core == 1).all() and kmax == 1
+            assert info.push_iters >= n // 2
+            G.tune_set("fusion_spin_limit", 0)
+            with pytest.raises(G.GprError, match="kcore"):
+                g.kcore()
+            G.tune_set("fusion_spin_limit", old)
+            core, kmax, _ = g.kcore()
+            assert (core == 1).all() and kmax == 1
+    finally:
+        G.tune_set("fusion_spin_limit", old)
