@@ -15,6 +15,9 @@ Also reported: per-algorithm GTEPS, the HBM roofline of the dominant step
 stream), BF16/FP16
 PR per-iteration GTEPS, and the CPU baseline (the OpenMP restatement of
 Graptor's algorithm in oracle/, timed on the host on a bounded sample).
+`apps` times the other two benchmarks on the same graph, each beside its
+CPU oracle, which also checks the results at full size: APR_vary (delta
+PageRank, variant 2046) and KC_bucket (coreness).
 """
 import argparse
 import json
@@ -184,13 +187,48 @@ def run(args):
                          "edgemap_GBs": pr_pull_bytes(n, m, s) / (pm * 1e-3) / 1e9,
                          "speedup_vs_fp32_iter": (info.edges / info.ms) / (ep / tp)}
         res["pr_precision"] = prec
+    outs = None
+    if not args.no_apps:
+        res["apps"], outs = gpu_apps(g, args)
     if not args.no_cpu:
-        res["cpu_baseline"], res["parity"] = cpu_baseline(g, args, src, lv, lab, x)
+        res["cpu_baseline"], res["parity"] = cpu_baseline(g, args, src, lv, lab, x, res.get("apps"), outs)
     g.destroy()
     return res
 
 
-def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu):
+def gpu_apps(g, args):
+    """APR_vary (variant 2046: FP32 contributions to L1 < 1e-1, then
+    scustomfp<true,6,9> to 1e-7, eps2 = 1e-2) and KC_bucket on the graph:
+    one warm-up run, `steps` timed runs leaving the results in HBM, then one
+    run that downloads them.  GTEPS counts edges as the reference does
+    (every edge of an all-sources iteration, the frontier's edges otherwise;
+    KC: the edges of the peeled vertices, i.e. m)."""
+    out = {}
+    res = {}
+    g.apr_vary(download=False)
+    ms = []
+    for _ in range(args.steps):
+        _, info = g.apr_vary(download=False)
+        ms.append(info.ms)
+    x, info = g.apr_vary()
+    res["apr_vary"] = {"device_ms": sum(ms) / len(ms), "gteps": info.edges / (sum(ms) / len(ms) * 1e-3) / 1e9,
+                       "iters": info.iters, "push_iters": info.push_iters, "edges": info.edges,
+                       "l1_final": info.delta}
+    out["apr"] = x
+    g.kcore(download=False)
+    ms = []
+    for _ in range(args.steps):
+        _, _, info = g.kcore(download=False)
+        ms.append(info.ms)
+    core, kmax, info = g.kcore()
+    res["kc_bucket"] = {"device_ms": sum(ms) / len(ms), "gteps": info.edges / (sum(ms) / len(ms) * 1e-3) / 1e9,
+                        "levels": info.iters, "rounds": info.push_iters, "largest_core": kmax}
+    out["core"] = (core, kmax)
+    log("apps: apr %.2f ms, kcore %.2f ms" % (res["apr_vary"]["device_ms"], res["kc_bucket"]["device_ms"]))
+    return res, out
+
+
+def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu, apps=None, outs=None):
     """The OpenMP restatement of Graptor's algorithm (oracle/) on the host
     cores: one BFSLVLv run, one CCv run and a few PRv iterations on the same
     graph.  Its results double as a full-size parity check of the GPU run."""
@@ -240,6 +278,24 @@ def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu):
     np.minimum.at(minorig, lab, order)
     parity = {"bfs_levels_equal": bool(np.array_equal(lv, lv_gpu[order])),
               "cc_labels_equal": bool(np.array_equal(minorig[lab], lab_gpu[order]))}
+    if apps is not None and outs is not None:
+        # APR_vary: the OpenMP restatement on the same graph; KC_bucket: the
+        # textbook bucket algorithm (one core)
+        t = time.perf_counter()
+        xa, it, l1, ed, _ = O.apr_vary(index, nbrs, threads=threads)
+        t_apr = time.perf_counter() - t
+        log("cpu apr_vary %d iters %.2fs" % (it, t_apr))
+        apps["apr_vary"]["cpu"] = {"seconds": t_apr, "gteps": ed / t_apr / 1e9, "cores": threads, "kind": "port",
+                                   "iters": it}
+        parity["apr_vary_l1_vs_oracle"] = float(np.abs(outs["apr"][order] - xa).sum())
+        t = time.perf_counter()
+        core, kmax = O.kcore(index, nbrs)
+        t_kc = time.perf_counter() - t
+        log("cpu kcore %.2fs" % t_kc)
+        apps["kc_bucket"]["cpu"] = {"seconds": t_kc, "gteps": m / t_kc / 1e9, "cores": 1, "kind": "port",
+                                    "algorithm": "sequential bucket peeling (Batagelj-Zaversnik)"}
+        parity["kc_coreness_equal"] = bool(np.array_equal(outs["core"][0][order], core)
+                                           and outs["core"][1] == kmax)
     return base, parity
 
 
@@ -266,6 +322,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-reorder", action="store_true")
+    ap.add_argument("--no-apps", action="store_true", help="skip the APR_vary and KC_bucket timings")
     ap.add_argument("--pr-split-bytes", type=float, default=None,
                     help="override the tunable pr_split_bytes (FP32 bytes of pulled sources; 0 = no split)")
     args = ap.parse_args()
