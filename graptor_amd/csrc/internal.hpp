@@ -117,6 +117,7 @@ constexpr uint64_t COLD_WCHUNK = 262144;   // pairs per write chunk
 constexpr uint64_t COLD_CHUNK_E = 1u << 20;
 constexpr uint64_t COLD_PAD_E = 32768;          // zero tail of the destination array (and of the pair bases / 64)
 constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
+constexpr uint32_t COLD_GROUP = 16;             // chunk partials summed per first-level combine group
 struct ColdLayout {
     uint32_t from = 0;              // first cold source
     uint32_t ntiles = 0, nbins = 0;
@@ -132,8 +133,11 @@ struct ColdLayout {
     uint32_t nwchunks = 0, nachunks = 0, nmulti = 0;
     uint32_t *wchunk = nullptr;     // per write chunk: tile, first pair, end pair (tile-major), first chunk of tile
     uint32_t *achunk = nullptr;     // per accumulation chunk: bin, first entry, end entry (bin-major), slot
-    uint32_t *multi = nullptr;      // per bin in several chunks: bin, first slot, chunks, 0
+    uint32_t *multi = nullptr;      // per bin in several chunks: bin, first slot, chunks, first group slot
     long long *partials = nullptr;  // COLD_BIN fixed-point sums per slot
+    uint32_t ngroups = 0;
+    uint32_t *groups = nullptr;     // per group of <= COLD_GROUP slots of one bin: first slot, slots, group slot, 0
+    long long *partials2 = nullptr; // COLD_BIN fixed-point sums per group
     double *mass = nullptr;         // per write chunk: sum over its tile's edges of |contribution| (first chunk)
 };
 

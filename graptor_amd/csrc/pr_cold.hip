@@ -27,7 +27,10 @@
 //                   not depend on the order of the atomics.  A bin in one
 //                   chunk updates y directly; the chunks of a heavy bin (the
 //                   hub destinations) write integer partials that
-//   k_cold_combine  adds up, per destination.
+//   k_cold_combine_groups and k_cold_combine add up per destination in two
+//                   levels (groups of COLD_GROUP partials, then the group
+//                   sums), so a hub bin of hundreds of chunks is summed by
+//                   many workgroups instead of one dependent chain each.
 //
 // After degree relabelling a source's edges fall into few bins relative to
 // its degree (RMAT: 0.10 / 0.16 / 0.22 pairs per edge at scales 20 / 22 /
@@ -192,6 +195,8 @@ void cold_free(gpr_graph *g) {
     dev_free(L->achunk);
     dev_free(L->multi);
     dev_free(L->partials);
+    dev_free(L->groups);
+    dev_free(L->partials2);
     delete L;
     g->cold = nullptr;
 }
@@ -278,8 +283,8 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     void *tmp = nullptr;
     int r = GPR_OK;
     std::vector<uint32_t> ecnt, pcnt, edel, pdel;
-    std::vector<uint32_t> wch, ach, multi;   // plans, 4 words per item
-    uint32_t nslots = 0;
+    std::vector<uint32_t> wch, ach, multi, groups;   // plans, 4 words per item
+    uint32_t nslots = 0, nslots2 = 0;
     uint64_t npairs = 0;
     const unsigned pg = grid_for(ncold, CK_EDGES, 65536);
     const unsigned eg = grid_for(ncold, 256, 65536);
@@ -372,10 +377,18 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
                 ach.push_back(k == 1 ? COLD_DIRECT : nslots + i);
             }
             if (k > 1) {
+                // the combine adds a hub bin's k chunk partials in two levels:
+                // groups of <= COLD_GROUP slots, then the groups' sums
                 multi.push_back(b);
                 multi.push_back(nslots);
                 multi.push_back(k);
-                multi.push_back(0);
+                multi.push_back(nslots2);
+                for (uint32_t g0 = 0; g0 < k; g0 += COLD_GROUP) {
+                    groups.push_back(nslots + g0);
+                    groups.push_back(std::min<uint32_t>(COLD_GROUP, k - g0));
+                    groups.push_back(nslots2++);
+                    groups.push_back(0);
+                }
                 nslots += k;
             }
         }
@@ -427,14 +440,20 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         L->nwchunks = (uint32_t)(wch.size() / 4);
         L->nachunks = (uint32_t)(ach.size() / 4);
         L->nmulti = (uint32_t)(multi.size() / 4);
+        L->ngroups = (uint32_t)(groups.size() / 4);
         if ((r = dev_alloc((void **)&L->wchunk, sizeof(uint32_t) * wch.size(), "cold write chunks"))) break;
         if ((r = dev_alloc((void **)&L->achunk, sizeof(uint32_t) * ach.size(), "cold chunks"))) break;
         if ((r = dev_alloc((void **)&L->mass, sizeof(double) * L->nwchunks, "cold mass"))) break;
         if (L->nmulti) {
             if ((r = dev_alloc((void **)&L->multi, sizeof(uint32_t) * multi.size(), "cold multi-chunk bins"))) break;
             if ((r = dev_alloc((void **)&L->partials, sizeof(long long) * COLD_BIN * nslots, "cold partials"))) break;
+            if ((r = dev_alloc((void **)&L->groups, sizeof(uint32_t) * groups.size(), "cold combine groups"))) break;
+            if ((r = dev_alloc((void **)&L->partials2, sizeof(long long) * COLD_BIN * nslots2, "cold group partials")))
+                break;
             if (hipMemcpy(L->multi, multi.data(), sizeof(uint32_t) * multi.size(), hipMemcpyHostToDevice) !=
-                hipSuccess) { r = GPR_E_HIP; break; }
+                    hipSuccess ||
+                hipMemcpy(L->groups, groups.data(), sizeof(uint32_t) * groups.size(), hipMemcpyHostToDevice) !=
+                    hipSuccess) { r = GPR_E_HIP; break; }
         }
         if (hipMemcpy(L->wchunk, wch.data(), sizeof(uint32_t) * wch.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(L->achunk, ach.data(), sizeof(uint32_t) * ach.size(), hipMemcpyHostToDevice) != hipSuccess) {
@@ -748,17 +767,36 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
 constexpr int CC_THREADS = 1024;
 constexpr uint32_t CC_PER_BIN = COLD_BIN / CC_THREADS;
 
-// Bins in several chunks: one workgroup per CC_THREADS destinations sums
-// the chunks' integer partials; its y sum and y/deg maximum go to slot
-// nbins + blockIdx.x.
+// Bins in several chunks, first level: one workgroup per (group of <=
+// COLD_GROUP chunk partials, CC_THREADS destinations) adds the group's
+// partials, all loads in flight together, into the group's slot.
 __global__ void __launch_bounds__(CC_THREADS)
-k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, const long long *__restrict__ partials,
+k_cold_combine_groups(const uint32_t *__restrict__ groups, const long long *__restrict__ partials,
+                      long long *__restrict__ partials2) {
+    const uint32_t gi = blockIdx.x / CC_PER_BIN, sub = blockIdx.x % CC_PER_BIN;
+    const uint32_t first = groups[4 * gi], cnt = groups[4 * gi + 1], slot2 = groups[4 * gi + 2];
+    const uint32_t i = sub * CC_THREADS + threadIdx.x;
+    long long t[COLD_GROUP];
+#pragma unroll
+    for (uint32_t c = 0; c < COLD_GROUP; ++c) t[c] = c < cnt ? partials[(size_t)(first + c) * COLD_BIN + i] : 0ll;
+    long long s = 0;
+#pragma unroll
+    for (uint32_t c = 0; c < COLD_GROUP; ++c) s += t[c];
+    partials2[(size_t)slot2 * COLD_BIN + i] = s;
+}
+
+// Second level: one workgroup per CC_THREADS destinations of a bin adds its
+// groups' sums and stores y; its y sum and y/deg maximum go to slot
+// nbins + blockIdx.x.  Integer sums: exact in any grouping.
+__global__ void __launch_bounds__(CC_THREADS)
+k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, const long long *__restrict__ partials2,
                const uint32_t *__restrict__ deg, float *__restrict__ y, int add, double *__restrict__ part,
                float *__restrict__ part_max, const PrScalars *__restrict__ prs, int f16) {
     __shared__ double dred[CC_THREADS / 64];
     __shared__ float fred[CC_THREADS / 64];
     const uint32_t mb = blockIdx.x / CC_PER_BIN, sub = blockIdx.x % CC_PER_BIN;
-    const uint32_t b = multi[4 * mb], first = multi[4 * mb + 1], k = multi[4 * mb + 2];
+    const uint32_t b = multi[4 * mb], first = multi[4 * mb + 3];
+    const uint32_t k = (multi[4 * mb + 2] + COLD_GROUP - 1) / COLD_GROUP;
     const uint32_t i = sub * CC_THREADS + threadIdx.x;
     const uint32_t d0 = b * COLD_BIN, nd = min(COLD_BIN, n - d0);
     double my_sum = 0.0;
@@ -769,11 +807,11 @@ k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, c
         for (; c + 8 <= k; c += 8) {
             long long t[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) t[j] = partials[(size_t)(first + c + j) * COLD_BIN + i];
+            for (int j = 0; j < 8; ++j) t[j] = partials2[(size_t)(first + c + j) * COLD_BIN + i];
 #pragma unroll
             for (int j = 0; j < 8; ++j) s += t[j];
         }
-        for (; c < k; ++c) s += partials[(size_t)(first + c) * COLD_BIN + i];
+        for (; c < k; ++c) s += partials2[(size_t)(first + c) * COLD_BIN + i];
         cold_store(y, d0 + i, s, prs->cold_unit, add, my_sum, my_max, deg, f16 != 0);
     }
     block_emit<CC_THREADS>(my_sum, my_max, part, part_max, nbins + blockIdx.x, dred, fred);
@@ -814,10 +852,13 @@ int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, fl
     case GPR_CONTRIB_SCFP16: CA(GPR_CONTRIB_SCFP16); break;
     }
 #undef CA
-    if (L->nmulti)
+    if (L->nmulti) {
+        hipLaunchKernelGGL(k_cold_combine_groups, dim3(L->ngroups * CC_PER_BIN), dim3(CC_THREADS), 0, g->stream,
+                           L->groups, (const long long *)L->partials, L->partials2);
         hipLaunchKernelGGL(k_cold_combine, dim3(L->nmulti * CC_PER_BIN), dim3(CC_THREADS), 0, g->stream, L->multi,
-                           L->nbins, g->n, (const long long *)L->partials, g->deg, y, add, part, part_max, g->prs,
+                           L->nbins, g->n, (const long long *)L->partials2, g->deg, y, add, part, part_max, g->prs,
                            fmt == GPR_CONTRIB_F16 ? 1 : 0);
+    }
     GPR_HIP(hipGetLastError());
     *nparts = L->nbins + L->nmulti * CC_PER_BIN;
     return GPR_OK;
