@@ -13,17 +13,22 @@
 // Here the whole peeling is ONE cooperative launch (the levels are many and
 // mostly small: hundreds of distinct core values and several rounds each
 // on RMAT graphs), with grid barriers between its phases:
-//   level K, pass A  minimum count over the candidates (vertices not yet
-//                    peeled): each wave owns one segment of them for the
-//                    whole launch (level 0: a range of vertex IDs);
-//   level K, pass B  candidates with count K form the frontier, those above
-//                    K stay in the segment, compacted in place (the count
-//                    lives in the wave's registers: no atomics), peeled ones
-//                    drop;
-//   round            the frontier's out-edges count their destinations down;
-//                    the one decrement that takes a destination from K+1 to
-//                    K appends it to the next round's frontier, so every
-//                    vertex is listed once and no dedup bitmap is needed.
+//   pass     over the candidates (vertices not yet peeled; each wave owns
+//            one segment of them for the whole launch, the first pass a
+//            range of vertex IDs): count K -> the level's frontier, above
+//            K -> kept, compacted in place (the count lives in the wave's
+//            registers: no atomics), with the smallest kept count;
+//   round    the frontier's out-edges count their destinations down; the
+//            one decrement that takes a destination from K+1 to K appends
+//            it to the next round's frontier, so every vertex is listed
+//            once and no dedup bitmap is needed.  Counts decremented to
+//            above K are folded into a minimum as well.
+// The next level's K is the smaller of the two minima: never above the
+// smallest remaining count (every vertex not peeled either kept its count
+// since the pass or was last seen at a decremented one), and below it only
+// when a vertex counted down was then peeled; the pass then finds no vertex
+// at K, no round runs, and its own kept minimum is exact.  So no separate
+// scan for the minimum is needed.
 // The count-down is a plain peek (counts only fall, so a stale value that is
 // at most K already proves there is nothing to do), then atomicSub; a
 // decrement that finds the count at or below K (a race with another
@@ -61,9 +66,9 @@ constexpr uint32_t KC_HUBS = 8192;      // count-downs of vertices below this ID
 static_assert(KC_HUBS % KC_THREADS == 0, "whole passes over the hub counters");
 constexpr unsigned KC_MAX_WAVES = GS_GROUP * GS_MAX_GROUPS * KC_WAVES;
 static_assert(KC_SLICE == 64 * KC_U, "a slice is one pass of the wave");
-// phase kinds timed by block 0: pass A, its barrier, pass B, its barrier,
-// round work, round barriers
-constexpr int KC_PROF = 6;
+// phase kinds timed by block 0: the candidate pass, its barrier, round
+// work, round barriers
+constexpr int KC_PROF = 4;
 
 struct KcCtl {
     unsigned long long arrive;        // groups arrived at the barrier (monotonic)
@@ -74,8 +79,9 @@ struct KcCtl {
     unsigned int levels;              // levels with a non-empty frontier
     unsigned int rounds;              // rounds run
     unsigned int kmax;                // largest core number
-    unsigned int kmin[2];             // level minima, alternating
-    unsigned long long pad[GS_LINE - 11];
+    unsigned int kmin_s[3];           // minimum count kept by a pass, rotating per iteration
+    unsigned int kmin_d[3];           // minimum count a round decremented to above K, rotating
+    unsigned long long pad[GS_LINE - 13];
     unsigned long long group[GS_MAX_GROUPS * GS_LINE];
     unsigned long long prof[KC_PROF];   // block 0's wall-clock ticks per phase kind
 };
@@ -171,7 +177,7 @@ __device__ __forceinline__ void kc_append(const bool (&put)[KC_U], const uint32_
 __device__ __forceinline__ void kc_count_down(const bool (&act)[KC_U], const uint32_t (&d)[KC_U], uint32_t K,
                                               uint32_t *core, const uint32_t *__restrict__ deg, const KcLists &L,
                                               int nb, unsigned long long *nlight, unsigned long long *nheavy,
-                                              KcStage &st, uint32_t *hub) {
+                                              KcStage &st, uint32_t *hub, uint32_t &dmin) {
     bool need[KC_U];
 #pragma unroll
     for (int j = 0; j < KC_U; ++j) need[j] = act[j] && core[d[j]] > K;
@@ -190,6 +196,7 @@ __device__ __forceinline__ void kc_count_down(const bool (&act)[KC_U], const uin
     for (int j = 0; j < KC_U; ++j) {
         if (need[j] && old[j] <= K) atomicAdd(&core[d[j]], 1u);   // raced at the floor: undo
         put[j] = need[j] && old[j] == K + 1;
+        if (need[j] && old[j] > K + 1 && old[j] - 1 < dmin) dmin = old[j] - 1;
     }
     kc_append(put, d, deg, L, nb, nlight, nheavy, st);
 }
@@ -201,7 +208,8 @@ __device__ __forceinline__ void kc_count_down(const bool (&act)[KC_U], const uin
 // and level) appends the hub.  Block-uniform: every thread calls it.
 __device__ __forceinline__ void kc_hub_flush(uint32_t *hub, uint32_t n, uint32_t K, uint32_t *core,
                                              const uint32_t *__restrict__ deg, const KcLists &L, int nb,
-                                             unsigned long long *nlight, unsigned long long *nheavy, KcStage &st) {
+                                             unsigned long long *nlight, unsigned long long *nheavy, KcStage &st,
+                                             uint32_t &dmin) {
     __syncthreads();
     for (uint32_t base = 0; base < KC_HUBS; base += KC_THREADS) {
         const uint32_t j = base + threadIdx.x;
@@ -216,6 +224,8 @@ __device__ __forceinline__ void kc_hub_flush(uint32_t *hub, uint32_t n, uint32_t
             } else if (old - dl <= K) {
                 if (old - dl < K) atomicAdd(&core[j], K - (old - dl));
                 put[0] = true;
+            } else if (old - dl < dmin) {
+                dmin = old - dl;
             }
         }
         kc_append(put, v, deg, L, nb, nlight, nheavy, st);
@@ -244,8 +254,16 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
     uint32_t cnt = (uint32_t)(wid * S >= n ? 0ull : ((unsigned long long)n - wid * S < S ? n - wid * S : S));
     unsigned ph = 0;          // barriers passed
     unsigned r = 0;           // rounds run
-    uint32_t lo = 0;          // vertices not yet peeled have a count >= lo
+    unsigned levels = 0;      // iterations with a non-empty frontier
+    // K, the count of this iteration's frontier, never exceeds the smallest
+    // count of a vertex not yet peeled: it is the minimum of the counts the
+    // last pass kept and of those the last level's rounds decremented to
+    // above its floor.  A vertex counted there and then peeled in the same
+    // level can make it too small; the pass then finds no vertex at K, no
+    // round runs, and the minimum it kept is exact.
+    uint32_t K = 0;
     unsigned long long my_edges = 0;
+    __shared__ uint32_t s_dmin, s_dflushed;
     // block 0's phase times live in LDS (registers are the scarce resource)
     __shared__ unsigned long long s_prof[KC_PROF + 1];
     if (leader) {
@@ -260,38 +278,23 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
         }
     };
     bool ok = true;
-    for (unsigned lvl = 0; ok; ++lvl) {
-        const int lb = lvl & 1, nlb = (lvl + 1) & 1;
+    for (unsigned it = 0; ok; ++it) {
+        const int s3 = it % 3;
         if (leader) {
-            // last read in level lvl-1, next written after this level's
-            // first barrier
-            __hip_atomic_store(&ctl->kmin[nlb], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // slot (it+1)%3 was last read after iteration it-2's last
+            // barrier, so before iteration it-1's first; it is written next
+            // in iteration it+1, after this iteration's barriers
+            __hip_atomic_store(&ctl->kmin_s[(it + 1) % 3], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->kmin_d[(it + 1) % 3], ~0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        // pass A: the smallest count among this wave's candidates
-        uint32_t mn = ~0u;
-        for (uint32_t base = 0; base < cnt; base += 64 * KC_PU) {
-            uint32_t v[KC_U] = {0u, 0u, 0u, 0u}, c[KC_U];
-#pragma unroll
-            for (int u = 0; u < KC_PU; ++u) {
-                const uint32_t i = base + u * 64 + ln;
-                v[u] = i >= cnt ? 0u : lvl == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
-            }
-#pragma unroll
-            for (int u = 0; u < KC_PU; ++u) c[u] = base + u * 64 + ln < cnt ? ld32(&core[v[u]]) : ~0u;
-#pragma unroll
-            for (int u = 0; u < KC_PU; ++u)
-                if (c[u] >= lo && c[u] < mn) mn = c[u];
+        if (threadIdx.x == 0) {
+            s_dmin = ~0u;
+            s_dflushed = ~0u;
         }
-        mn = wave_min(mn);
-        if (ln == 0 && mn != ~0u)
-            __hip_atomic_fetch_min(&ctl->kmin[lb], mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        mark(0);
-        if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) break;
-        mark(1);
-        const uint32_t K = __hip_atomic_load(&ctl->kmin[lb], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (K == ~0u) break;                       // every vertex peeled
-        // pass B: count K -> this level's frontier; above K -> kept in the
-        // segment (written at or below the position read, so in place)
+        // the pass: count K -> this level's frontier; above K -> kept in the
+        // segment (written at or below the position read, so in place), and
+        // their smallest count; below K -> peeled before, dropped
+        uint32_t smin = ~0u;
         {
             const int c3 = r % 3, cb = r & 1;
             uint32_t p = 0;
@@ -300,16 +303,17 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
 #pragma unroll
                 for (int u = 0; u < KC_PU; ++u) {
                     const uint32_t i = base + u * 64 + ln;
-                    v[u] = i >= cnt ? 0u : lvl == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
+                    v[u] = i >= cnt ? 0u : it == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
                 }
 #pragma unroll
                 for (int u = 0; u < KC_PU; ++u) c[u] = base + u * 64 + ln < cnt ? ld32(&core[v[u]]) : 0u;
                 bool put[KC_U] = {false, false, false, false}, keep[KC_U];
 #pragma unroll
                 for (int u = 0; u < KC_PU; ++u) {
-                    const bool live = base + u * 64 + ln < cnt && c[u] >= lo;
-                    put[u] = live && c[u] == K;
-                    keep[u] = live && c[u] > K;
+                    const bool in = base + u * 64 + ln < cnt;
+                    put[u] = in && c[u] == K;
+                    keep[u] = in && c[u] > K;
+                    if (keep[u] && c[u] < smin) smin = c[u];
                 }
                 kc_append(put, v, deg, L, cb, &ctl->nlight[c3], &ctl->nheavy[c3], st);
                 // every read of these KC_PU chunks precedes the writes, which
@@ -325,10 +329,24 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
             cnt = p;
             kc_flush(st, L.light[cb], &ctl->nlight[c3]);
         }
-        mark(2);
+        smin = wave_min(smin);
+        if (ln == 0 && smin != ~0u)
+            __hip_atomic_fetch_min(&ctl->kmin_s[s3], smin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mark(0);
         if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) break;
-        mark(3);
+        mark(1);
+        {
+            const int c3 = r % 3;
+            if (ld_agent(&ctl->nlight[c3]) == 0 && ld_agent(&ctl->nheavy[c3]) == 0) {
+                // no vertex at K: no round runs, the kept counts are exact
+                const uint32_t ks = __hip_atomic_load(&ctl->kmin_s[s3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (ks == ~0u) break;              // every vertex peeled
+                K = ks;
+                continue;
+            }
+        }
         // rounds of level K
+        uint32_t dmin = ~0u;   // per lane: counts decremented to above K
         for (;;) {
             const int c3 = r % 3, cb = r & 1;
             const int n3 = (r + 1) % 3, nb = (r + 1) & 1;
@@ -360,7 +378,7 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                         d[j] = act[j] ? nbrs[e + j] : 0u;
                     }
                     e = e + KC_U < e1 ? e + KC_U : e1;
-                    kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub);
+                    kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub, dmin);
                 }
             }
             // heavy rows: one wave per slice
@@ -379,24 +397,41 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                     act[u] = e < ee;
                     d[u] = act[u] ? nbrs[e] : 0u;
                 }
-                kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub);
+                kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub, dmin);
             }
-            kc_hub_flush(s_hub, n, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st);
+            kc_hub_flush(s_hub, n, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, dmin);
             kc_flush(st, L.light[nb], &ctl->nlight[n3]);
+            // the block's smallest count decremented to above K, sent on when
+            // it improved (one global atomic per block and round at most)
+            {
+                const uint32_t wm = wave_min(dmin);
+                if (ln == 0 && wm != ~0u) atomicMin(&s_dmin, wm);
+                __syncthreads();
+                if (threadIdx.x == 0 && s_dmin < s_dflushed) {
+                    __hip_atomic_fetch_min(&ctl->kmin_d[s3], s_dmin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_dflushed = s_dmin;
+                }
+            }
             ++r;
-            mark(4);
+            mark(2);
             if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) {
                 ok = false;
                 break;
             }
-            mark(5);
+            mark(3);
         }
-        lo = K + 1;
+        if (!ok) break;
+        ++levels;
         if (leader) {
             __hip_atomic_store(&ctl->kmax, K, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&ctl->levels, lvl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&ctl->levels, levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             __hip_atomic_store(&ctl->rounds, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        const uint32_t ks = __hip_atomic_load(&ctl->kmin_s[s3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t kd = __hip_atomic_load(&ctl->kmin_d[s3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t kn = ks < kd ? ks : kd;
+        if (kn == ~0u) break;                      // nothing left above K
+        K = kn;
     }
     if (leader)
         for (int k = 0; k < KC_PROF; ++k)
@@ -413,9 +448,9 @@ __global__ void __launch_bounds__(256) k_kc_init(uint32_t n, const uint32_t *__r
         unsigned long long *w = (unsigned long long *)ctl;
         for (unsigned i = threadIdx.x; i < sizeof(KcCtl) / 8; i += 256) w[i] = 0;
         __syncthreads();
-        if (threadIdx.x == 0) {
-            ctl->kmin[0] = ~0u;
-            ctl->kmin[1] = ~0u;
+        if (threadIdx.x < 3) {
+            ctl->kmin_s[threadIdx.x] = ~0u;
+            ctl->kmin_d[threadIdx.x] = ~0u;
         }
     }
 }
@@ -504,12 +539,11 @@ int kcore_peel(gpr_graph *g, gpr_prop *core, KcStats *st) {
         int dev = 0, khz = 0;
         GPR_HIP(hipGetDevice(&dev));
         GPR_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-        static const char *names[KC_PROF] = {"kcore_pass_a", "kcore_wait_a", "kcore_pass_b",
-                                             "kcore_wait_b", "kcore_rounds", "kcore_wait_rounds"};
+        static const char *names[KC_PROF] = {"kcore_pass", "kcore_wait_pass", "kcore_rounds", "kcore_wait_rounds"};
         for (int k = 0; k < KC_PROF; ++k) {
             auto &e = g->prof.acc[names[k]];
             e.first += khz > 0 ? (double)t[k] / (double)khz : 0.0;
-            e.second += k < 4 ? h.levels : h.rounds;
+            e.second += k < 2 ? h.levels : h.rounds;
         }
     }
     st->kmax = g->n ? h.kmax : 0;
