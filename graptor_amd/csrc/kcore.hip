@@ -14,8 +14,10 @@
 // mostly small: hundreds of distinct core values and several rounds each
 // on RMAT graphs), with grid barriers between its phases:
 //   pass     over the candidates (vertices not yet peeled; each wave owns
-//            one segment of them for the whole launch, the first pass a
-//            range of vertex IDs): count K -> the level's frontier, above
+//            one segment of them for the whole launch, taken in the first
+//            pass as 64-vertex chunks dealt round-robin over the waves, so
+//            that the hubs, which survive longest, are spread over all of
+//            them): count K -> the level's frontier, above
 //            K -> kept, compacted in place (the count lives in the wave's
 //            registers: no atomics), with the smallest kept count;
 //   round    the frontier's out-edges count their destinations down; the
@@ -123,6 +125,31 @@ __device__ __forceinline__ void kc_flush(KcStage &st, uint32_t *list, unsigned l
     for (uint32_t i = lane(); i < st.n; i += 64)
         __hip_atomic_store(&list[b + i], st.buf[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_wave_barrier();
+    st.n = 0;
+}
+
+// The end-of-phase flush of every wave's stage with one reservation per
+// block: the waves' counts are prefixed in LDS and thread 0 reserves their
+// total.  Block-uniform: every thread calls it.
+__device__ __forceinline__ void kc_flush_block(KcStage &st, uint32_t *list, unsigned long long *nlight,
+                                               uint32_t *s_cnt, unsigned long long *s_base) {
+    const unsigned w = threadIdx.x / 64;
+    if (lane() == 0) s_cnt[w] = st.n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long tot = 0;
+        for (int i = 0; i < KC_WAVES; ++i) tot += s_cnt[i];
+        unsigned long long b = 0;
+        if (tot) b = __hip_atomic_fetch_add(nlight, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = 0; i < KC_WAVES; ++i) {
+            s_base[i] = b;
+            b += s_cnt[i];
+        }
+    }
+    __syncthreads();
+    const unsigned long long b = s_base[w];
+    for (uint32_t i = lane(); i < st.n; i += 64)
+        __hip_atomic_store(&list[b + i], st.buf[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     st.n = 0;
 }
 
@@ -246,12 +273,16 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
     const unsigned long long wid = (unsigned long long)(threadIdx.x / 64) * gridDim.x + blockIdx.x;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
     KcStage st = {&s_stage[(threadIdx.x / 64) * KC_STAGE], 0u};
-    // this wave's segment of the candidates: level 0 takes the vertices
-    // [wid*S, wid*S + S) without a list; each pass B compacts the survivors
-    // in place, so the count stays in a register and needs no atomic
-    const unsigned long long S = ((unsigned long long)n + W - 1) / W;
+    // this wave's segment of the candidates.  The first pass takes, without
+    // a list, the 64-vertex chunks wid, wid+W, wid+2W, ... (coalesced, and
+    // every wave gets a share of every degree range: after relabelling by
+    // degree the low IDs are the hubs, which survive to the highest levels);
+    // each pass compacts the survivors in place, so the count stays in a
+    // register and needs no atomic
+    const unsigned long long nchunks = ((unsigned long long)n + 63) / 64;
+    const unsigned long long S = (nchunks + W - 1) / W * 64;
     uint32_t *seg = L.cand + wid * S;
-    uint32_t cnt = (uint32_t)(wid * S >= n ? 0ull : ((unsigned long long)n - wid * S < S ? n - wid * S : S));
+    uint32_t cnt = (uint32_t)(wid < nchunks ? (nchunks - wid + W - 1) / W * 64 : 0ull);
     unsigned ph = 0;          // barriers passed
     unsigned r = 0;           // rounds run
     unsigned levels = 0;      // iterations with a non-empty frontier
@@ -263,7 +294,9 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
     // round runs, and the minimum it kept is exact.
     uint32_t K = 0;
     unsigned long long my_edges = 0;
-    __shared__ uint32_t s_dmin, s_dflushed;
+    __shared__ uint32_t s_dmin, s_dflushed, s_smin;
+    __shared__ uint32_t s_wcnt[KC_WAVES];
+    __shared__ unsigned long long s_wbase[KC_WAVES];
     // block 0's phase times live in LDS (registers are the scarce resource)
     __shared__ unsigned long long s_prof[KC_PROF + 1];
     if (leader) {
@@ -290,6 +323,7 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
         if (threadIdx.x == 0) {
             s_dmin = ~0u;
             s_dflushed = ~0u;
+            s_smin = ~0u;
         }
         // the pass: count K -> this level's frontier; above K -> kept in the
         // segment (written at or below the position read, so in place), and
@@ -300,19 +334,27 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
             uint32_t p = 0;
             for (uint32_t base = 0; base < cnt; base += 64 * KC_PU) {
                 uint32_t v[KC_U] = {0u, 0u, 0u, 0u}, c[KC_U];
+                bool in[KC_U] = {false, false, false, false};
 #pragma unroll
                 for (int u = 0; u < KC_PU; ++u) {
                     const uint32_t i = base + u * 64 + ln;
-                    v[u] = i >= cnt ? 0u : it == 0 ? (uint32_t)(wid * S) + i : ld32(&seg[i]);
+                    if (it == 0) {
+                        // chunk wid + (i/64)*W, vertex i%64 of it (the last chunk may be short)
+                        const unsigned long long vv = 64 * (wid + (unsigned long long)(i / 64) * W) + ln;
+                        in[u] = i < cnt && vv < n;
+                        v[u] = in[u] ? (uint32_t)vv : 0u;
+                    } else {
+                        in[u] = i < cnt;
+                        v[u] = in[u] ? ld32(&seg[i]) : 0u;
+                    }
                 }
 #pragma unroll
-                for (int u = 0; u < KC_PU; ++u) c[u] = base + u * 64 + ln < cnt ? ld32(&core[v[u]]) : 0u;
+                for (int u = 0; u < KC_PU; ++u) c[u] = in[u] ? ld32(&core[v[u]]) : 0u;
                 bool put[KC_U] = {false, false, false, false}, keep[KC_U];
 #pragma unroll
                 for (int u = 0; u < KC_PU; ++u) {
-                    const bool in = base + u * 64 + ln < cnt;
-                    put[u] = in && c[u] == K;
-                    keep[u] = in && c[u] > K;
+                    put[u] = in[u] && c[u] == K;
+                    keep[u] = in[u] && c[u] > K;
                     if (keep[u] && c[u] < smin) smin = c[u];
                 }
                 kc_append(put, v, deg, L, cb, &ctl->nlight[c3], &ctl->nheavy[c3], st);
@@ -327,11 +369,14 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                 }
             }
             cnt = p;
-            kc_flush(st, L.light[cb], &ctl->nlight[c3]);
+            // the block's minimum through LDS, then one global atomic; the
+            // flush's barriers order the LDS minimum before thread 0 reads it
+            smin = wave_min(smin);
+            if (ln == 0 && smin != ~0u) atomicMin(&s_smin, smin);
+            kc_flush_block(st, L.light[cb], &ctl->nlight[c3], s_wcnt, s_wbase);
         }
-        smin = wave_min(smin);
-        if (ln == 0 && smin != ~0u)
-            __hip_atomic_fetch_min(&ctl->kmin_s[s3], smin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0 && s_smin != ~0u)
+            __hip_atomic_fetch_min(&ctl->kmin_s[s3], s_smin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         mark(0);
         if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, ph++, spin_limit, &s_ok)) break;
         mark(1);
@@ -400,7 +445,7 @@ k_kcore(uint32_t n, const uint64_t *__restrict__ index, const uint32_t *__restri
                 kc_count_down(act, d, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, s_hub, dmin);
             }
             kc_hub_flush(s_hub, n, K, core, deg, L, nb, &ctl->nlight[n3], &ctl->nheavy[n3], st, dmin);
-            kc_flush(st, L.light[nb], &ctl->nlight[n3]);
+            kc_flush_block(st, L.light[nb], &ctl->nlight[n3], s_wcnt, s_wbase);
             // the block's smallest count decremented to above K, sent on when
             // it improved (one global atomic per block and round at most)
             {
@@ -467,7 +512,7 @@ static KcBuf kc_layout(const gpr_graph *g) {
     b.nheavy = n + g->m / KC_SLICE + 64;
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     b.cand = 0;
-    b.light = al((n + KC_MAX_WAVES) * sizeof(uint32_t));
+    b.light = al((n + 64 * (size_t)KC_MAX_WAVES) * sizeof(uint32_t));
     b.heavy = b.light + al(2 * n * sizeof(uint32_t));
     b.total = b.heavy + 2 * b.nheavy * sizeof(uint64_t);
     return b;
