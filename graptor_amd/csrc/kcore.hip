@@ -64,7 +64,7 @@ constexpr uint32_t KC_SLICE = 256;
 constexpr int KC_U = 4;
 constexpr int KC_STAGE = 512;           // light appends staged per wave in LDS
 constexpr int KC_PU = 4;                // candidate chunks per wave step in passes A and B
-constexpr uint32_t KC_HUBS = 8192;      // count-downs of vertices below this ID add up in LDS
+constexpr uint32_t KC_HUBS = 24576;      // count-downs of vertices below this ID add up in LDS
 static_assert(KC_HUBS % KC_THREADS == 0, "whole passes over the hub counters");
 constexpr unsigned KC_MAX_WAVES = GS_GROUP * GS_MAX_GROUPS * KC_WAVES;
 static_assert(KC_SLICE == 64 * KC_U, "a slice is one pass of the wave");
