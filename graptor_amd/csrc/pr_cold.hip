@@ -482,7 +482,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
 
 // ---- per-iteration kernels -------------------------------------------------
 constexpr int CW_THREADS = 1024;
-constexpr int CW_UNROLL = 8;
+constexpr int CW_UNROLL = 16;
 constexpr int CW_STAGE = 8;               // window loads per thread in flight
 constexpr uint32_t CW_DELTA_LDS = 4096;   // run offsets staged in LDS up to this many bins
 
