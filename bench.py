@@ -58,7 +58,7 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 # in and 8n out: FETCH_SIZE x 2.00, WRITE_SIZE x 1.00.  The summary with the
 # commands is profiles/pr_edgemap_rmat26_pmc.md.  Other configurations
 # report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 13.656e9}
+MEASURED_TRAFFIC = {(26, 16, 0.0): 13.619e9}
 
 
 def pr_edgemap_ms(g):
