@@ -711,8 +711,15 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
                                                                     BUF_NT);
             // the base was loaded an iteration ago; readfirstlane makes it
             // scalar (all lanes hold the same value)
-            const uint32_t pb = max(__builtin_amdgcn_readfirstlane(pbn[u]), pfirst);
-            vv[u] = buf_load(vrs, sizeof(S) * ln, sizeof(S) * (pb - pfirst), S());
+            // The offset is uniform by construction; readfirstlane puts it in
+            // a scalar register, as the load's scalar offset must be (an
+            // offset left in a vector register, as a mixed signed/unsigned
+            // max() or a fused clamp-subtract produce, becomes a waterfall
+            // loop around the load)
+            const uint32_t pbr = (uint32_t)__builtin_amdgcn_readfirstlane(pbn[u]);
+            const uint32_t pb = pbr > pfirst ? pbr : pfirst;
+            const uint32_t soff = (uint32_t)__builtin_amdgcn_readfirstlane((int)(sizeof(S) * (pb - pfirst)));
+            vv[u] = buf_load(vrs, sizeof(S) * ln, soff, S());
         }
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(base + step + u * nw * 64) >> 6];
