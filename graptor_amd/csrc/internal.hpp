@@ -115,7 +115,7 @@ constexpr uint32_t COLD_TILE_V = 32768;
 constexpr uint64_t COLD_TILE_E = 1ull << 24;
 constexpr uint64_t COLD_WCHUNK = 262144;   // pairs per write chunk
 constexpr uint64_t COLD_CHUNK_E = 1u << 20;
-constexpr uint64_t COLD_PAD_E = 32768;          // zero tail of the destination array (and of the pair bases / 64)
+constexpr uint64_t COLD_PAD_E = 65536;          // zero tail of the destination array (and of the pair bases / 64)
 constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
 constexpr uint32_t COLD_GROUP = 16;             // chunk partials summed per first-level combine group
 struct ColdLayout {

@@ -689,7 +689,7 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
     const uint32_t *bb = bbase + (a0 >> 6);
     const uint32_t step = nw * 64 * CA_U;
-    static_assert(2 * nw_max * 64 * CA_U <= COLD_PAD_E, "zero tail shorter than two loop steps");
+    static_assert(3 * nw_max * 64 * CA_U <= COLD_PAD_E, "zero tail shorter than three loop steps");
     // buffer descriptors from the chunk's first entry and first pair: the
     // uniform part of each offset goes into the scalar offset.  A block past
     // the end of the pair-base array reads base 0 (its zero tail); bases are
@@ -699,48 +699,50 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     const uint32_t pfirst = __builtin_amdgcn_readfirstlane(bb[0]);
     const uint64_t vbytes = (npairs + 64 - pfirst) * (uint64_t)sizeof(S);
     const __amdgpu_buffer_rsrc_t vrs = buf_rsrc(vals + pfirst, vbytes < 0xffffffffull ? (uint32_t)vbytes : 0xffffffffu);
-    uint32_t pbn[CA_U];
-#pragma unroll
-    for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(w * 64 + u * nw * 64) >> 6];
-    for (uint32_t base = w * 64; base < len; base += step) {
-        uint32_t key[CA_U];
-        uint32_t vv[CA_U];
+    // Two steps in flight: the loads of the next step are issued before
+    // the shuffles and atomics of this one, so that a wave's LDS work
+    // overlaps its own memory latency (one workgroup per CU, four waves per
+    // SIMD, leave the registers for it).  The bases run two steps ahead.
+    uint32_t pbn[CA_U], key[CA_U], vv[CA_U];
+    auto issue = [&](uint32_t at, uint32_t *kk, uint32_t *vl) {
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            key[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(krs, (int)(2 * ln), (int)(2 * (base + u * nw * 64)),
-                                                                    BUF_NT);
-            // the base was loaded an iteration ago; readfirstlane makes it
-            // scalar (all lanes hold the same value)
-            // The offset is uniform by construction; readfirstlane puts it in
-            // a scalar register, as the load's scalar offset must be (an
+            kk[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(krs, (int)(2 * ln), (int)(2 * (at + u * nw * 64)),
+                                                                   BUF_NT);
+            // the base was loaded a step ago and is uniform by construction;
+            // readfirstlane puts it, and the offset derived from it, in
+            // scalar registers, as the load's scalar offset must be (an
             // offset left in a vector register, as a mixed signed/unsigned
             // max() or a fused clamp-subtract produce, becomes a waterfall
             // loop around the load)
             const uint32_t pbr = (uint32_t)__builtin_amdgcn_readfirstlane(pbn[u]);
             const uint32_t pb = pbr > pfirst ? pbr : pfirst;
             const uint32_t soff = (uint32_t)__builtin_amdgcn_readfirstlane((int)(sizeof(S) * (pb - pfirst)));
-            vv[u] = buf_load(vrs, sizeof(S) * ln, soff, S());
+            vl[u] = buf_load(vrs, sizeof(S) * ln, soff, S());
         }
+    };
+    // the bases are uniform: read through the constant address space they
+    // are scalar loads, counted apart from the vector loads, so waiting for
+    // them does not wait for the step in flight
+    const __attribute__((address_space(4))) uint32_t *bbs = (const __attribute__((address_space(4))) uint32_t *)bb;
+    auto bases = [&](uint32_t at) {
 #pragma unroll
-        for (int u = 0; u < CA_U; ++u) pbn[u] = bb[(base + step + u * nw * 64) >> 6];
-        // keep every load above issued before the first use below (the
-        // asm stops the loads from being sunk to their uses; the three
-        // arrays are not __restrict__ for this reason)
-        asm volatile("" ::: "memory");
-        __builtin_amdgcn_sched_barrier(0);
+        for (int u = 0; u < CA_U; ++u) pbn[u] = bbs[(at + u * nw * 64) >> 6];
+    };
+    // LDS atomics are exact whatever the order and whatever the number of
+    // lanes on one address (the hub destinations).  A block wholly inside
+    // the chunk needs no per-lane range test, and the sign is applied only
+    // in a wave with a negative value (PageRank's are not).
+    auto process = [&](uint32_t base, const uint32_t *kk, const uint32_t *vl) {
         // every lane's value first (the shuffles are in flight together),
         // then the atomics
         uint32_t vb[CA_U];
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            const uint64_t m = __ballot(key[u] & 0x8000u);
+            const uint64_t m = __ballot(kk[u] & 0x8000u);
             const uint32_t r = mbcnt(m >> 1);
-            vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)vv[u]);
+            vb[u] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(r << 2), (int)vl[u]);
         }
-        // LDS atomics are exact whatever the order and whatever the number
-        // of lanes on one address (the hub destinations).  A block wholly
-        // inside the chunk needs no per-lane range test, and the sign is
-        // applied only in a wave with a negative value (PageRank's are not).
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
             const uint32_t ob = base + u * nw * 64, o = ob + ln;
@@ -749,8 +751,30 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
             if (__builtin_expect(__ballot(v < 0.0f) != 0, 0)) fx = v < 0.0f ? (uint64_t)(-(long long)fx) : fx;
             bool in = true;
             if (!(ob >= 64 && ob + 64 <= len)) in = o >= lo && o < len;
-            if (in) atomicAdd(&acc[key[u] & (COLD_BIN - 1)], (unsigned long long)fx);
+            if (in) atomicAdd(&acc[kk[u] & (COLD_BIN - 1)], (unsigned long long)fx);
         }
+    };
+    // two register sets that take turns (unrolled by two, so that no set is
+    // copied into the other: a copy would wait for its loads); the asm
+    // keeps each step's loads issued before the other step's uses (it stops
+    // loads from being sunk to their uses; the arrays are not __restrict__
+    // for this reason)
+    uint32_t key2[CA_U], vv2[CA_U];
+    bases(w * 64);
+    issue(w * 64, key, vv);
+    bases(w * 64 + step);
+    for (uint32_t base = w * 64; base < len; base += 2 * step) {
+        issue(base + step, key2, vv2);
+        bases(base + 2 * step);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        process(base, key, vv);
+        if (base + step >= len) break;
+        issue(base + 2 * step, key, vv);
+        bases(base + 3 * step);
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_sched_barrier(0);
+        process(base + step, key2, vv2);
     }
     __syncthreads();
     const uint32_t d0 = b * COLD_BIN, nd = min(COLD_BIN, n - d0);
