@@ -124,8 +124,11 @@ struct ColdLayout {
     uint64_t ncold = 0;             // cold edges (entries)
     uint64_t npairs = 0;            // distinct (source, bin) pairs over the cold edges
     uint32_t *tile_lo = nullptr;    // ntiles+1 source boundaries
-    uint32_t *delta = nullptr;      // ntiles x nbins: bin-major pair slot - tile-major pair position of cell (t,b)
-    uint32_t *wsrc = nullptr;       // per pair, tile-major: bin << 15 | (source - tile_lo[t])
+    // cells (t,b) with pairs are numbered in tile-major order (ordinals)
+    uint32_t *roff = nullptr;       // per non-empty cell: bin-major pair slot - tile-major pair position
+    uint32_t *tcell = nullptr;      // ntiles+1: ordinal of each tile's first non-empty cell
+    uint16_t *wsrc = nullptr;       // per pair, tile-major: source - tile_lo[t], bit 15 = first pair of its cell
+    uint32_t *wbase = nullptr;      // per 64 tile-major pairs: ordinal of the first pair's cell
     uint16_t *cdst = nullptr;       // per entry, bin-major: destination - b*COLD_BIN, bit 15 = first entry of a pair;
                                     // COLD_PAD_E zero entries follow
     uint32_t *bbase = nullptr;      // per 64 bin-major entries: the pair of the first entry (zero tail)

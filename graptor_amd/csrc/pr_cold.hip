@@ -10,12 +10,16 @@
 //
 //   k_cold_write    per chunk of a tile of consecutive sources: the tile's
 //                   contributions are staged in LDS; its pairs are read in
-//                   (bin, source) order (a coalesced 4 B word: bin and
-//                   source offset) and each pair's value is written once to
-//                   its slot in the bin-major value array, where the tile's
-//                   pairs of one bin form a contiguous run.  The first chunk
-//                   of a tile also sums |contribution| x out-degree over the
-//                   tile's sources (the mass of its edges).
+//                   (bin, source) order (a coalesced 2 B word: the source
+//                   offset, bit 15 on the first pair of its (tile, bin)
+//                   cell) and each pair's value is written once to its slot
+//                   in the bin-major value array, where the tile's pairs of
+//                   one bin form a contiguous run.  A pair's cell is the
+//                   cell of its 64-pair block's first pair plus a ballot
+//                   prefix count of the cell starts, and the cell gives the
+//                   run's offset.  The first chunk of a tile also sums
+//                   |contribution| x out-degree over the tile's sources (the
+//                   mass of its edges).
 //   k_cold_accum    per chunk of a bin of COLD_BIN destinations: the bin's
 //                   entries (2 B destinations, one per edge, grouped by
 //                   pair) are streamed; bit 15 marks the first entry of a
@@ -35,7 +39,8 @@
 // After degree relabelling a source's edges fall into few bins relative to
 // its degree (RMAT: 0.10 / 0.16 / 0.22 pairs per edge at scales 20 / 22 /
 // 24), so a value crosses HBM once per pair instead of once per edge; the
-// per-edge traffic is the 2 B destination.  This is the "partition-centric"
+// per-edge traffic is the 2 B destination (and per pair a 2 B write word,
+// the 4 B value written and the 4 B value read back).  This is the "partition-centric"
 // form of propagation blocking (Lakhotia, Kannan and Prasanna, USENIX ATC
 // 2018).
 //
@@ -161,21 +166,26 @@ __global__ void k_cold_cellpairs(const uint32_t *__restrict__ cstart, uint64_t c
 
 // Sorted entry p, with incl = inclusive prefix sum of the flags: its
 // destination (and pair flag) at its bin-major slot p + edelta[cell]; a
-// pair's write word at the pair's tile-major position incl[p] - 1; and for
-// an entry on a 64-entry boundary of the bin-major order, the block's pair
-// base (the bin-major pair of that entry).
+// pair's write word (its source offset, bit 15 on the first pair of its
+// cell) at the pair's tile-major position incl[p] - 1, and for a pair on a
+// 64-pair boundary its cell's ordinal; and for an entry on a 64-entry
+// boundary of the bin-major order, the block's pair base (the bin-major
+// pair of that entry).
 __global__ void k_cold_finish(const uint32_t *__restrict__ keys, const uint32_t *__restrict__ words,
-                              const uint32_t *__restrict__ incl, uint64_t count, uint32_t nbins,
-                              const uint32_t *__restrict__ edelta, const uint32_t *__restrict__ pdelta,
-                              uint32_t *wsrc, uint16_t *cdst, uint32_t *bbase) {
+                              const uint32_t *__restrict__ incl, uint64_t count, const uint32_t *__restrict__ edelta,
+                              const uint32_t *__restrict__ pdelta, const uint32_t *__restrict__ ord, uint16_t *wsrc,
+                              uint32_t *wbase, uint16_t *cdst, uint32_t *bbase) {
     for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < count; p += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t cell = keys[p], w = words[p];
-        const uint32_t b = cell % nbins;
         const uint32_t q = incl[p] - 1;   // tile-major pair
         const uint32_t f = incl[p] - (p ? incl[p - 1] : 0u);
         const uint32_t E = (uint32_t)p + edelta[cell];   // modulo 2^32, as the offsets
         cdst[E] = (uint16_t)((w & (COLD_BIN - 1)) | (f << 15));
-        if (f) wsrc[q] = (b << 15) | (w >> 14);
+        if (f) {
+            const bool first = p == 0 || keys[p - 1] != cell;   // a cell's first entry starts a pair
+            wsrc[q] = (uint16_t)((w >> 14) | (first ? 0x8000u : 0u));
+            if ((q & 63u) == 0) wbase[q >> 6] = ord[cell];
+        }
         if ((E & 63u) == 0) bbase[E >> 6] = q + pdelta[cell];
     }
 }
@@ -185,8 +195,10 @@ void cold_free(gpr_graph *g) {
     if (!L) return;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     dev_free(L->tile_lo);
-    dev_free(L->delta);
+    dev_free(L->roff);
+    dev_free(L->tcell);
     dev_free(L->wsrc);
+    dev_free(L->wbase);
     dev_free(L->cdst);
     dev_free(L->bbase);
     dev_free(L->vals);
@@ -262,8 +274,6 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     const uint32_t n = g->n;
     if (from >= n) return GPR_OK;   // nothing cold: ntiles = 0
     const uint32_t nbins = (uint32_t)(((uint64_t)n + COLD_BIN - 1) / COLD_BIN);
-    // the write kernel's word holds the bin in 17 bits
-    if (nbins > (1u << 17)) return GPR_OK;
     std::vector<uint64_t> idx((size_t)(n - from) + 1);
     GPR_HIP(hipMemcpyAsync(idx.data(), g->index + from, sizeof(uint64_t) * idx.size(), hipMemcpyDeviceToHost,
                            g->stream));
@@ -280,9 +290,10 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     const size_t cells = (size_t)ntiles * nbins;
     if (cells >= 0xffffffffull) return GPR_OK;   // cell keys are 32-bit: the pull handles every edge
     uint32_t *cnt = nullptr, *k0 = nullptr, *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *edelta = nullptr;
+    uint32_t *pdelta = nullptr, *dord = nullptr;
     void *tmp = nullptr;
     int r = GPR_OK;
-    std::vector<uint32_t> ecnt, pcnt, edel, pdel;
+    std::vector<uint32_t> ecnt, pcnt, edel, pdel, ord, roff, tcell;
     std::vector<uint32_t> wch, ach, multi, groups;   // plans, 4 words per item
     uint32_t nslots = 0, nslots2 = 0;
     uint64_t npairs = 0;
@@ -398,6 +409,21 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
             plan_runs(pcnt, ntiles, nbins, pdel, pbstart);
             npairs = pbstart[nbins];
             uint64_t pos = 0;
+            // the non-empty cells in tile-major order: each cell's ordinal,
+            // the run offsets by ordinal, each tile's first ordinal (every
+            // tile has pairs)
+            ord.assign(cells, 0u);
+            tcell.assign(ntiles + 1, 0u);
+            for (uint32_t t = 0; t < ntiles; ++t) {
+                tcell[t] = (uint32_t)roff.size();
+                for (uint32_t b = 0; b < nbins; ++b) {
+                    const size_t cell = (size_t)t * nbins + b;
+                    if (!pcnt[cell]) continue;
+                    ord[cell] = (uint32_t)roff.size();
+                    roff.push_back(pdel[cell]);
+                }
+            }
+            tcell[ntiles] = (uint32_t)roff.size();
             for (uint32_t t = 0; t < ntiles; ++t) {
                 const uint64_t t0 = pos;
                 for (uint32_t b = 0; b < nbins; ++b) pos += pcnt[(size_t)t * nbins + b];
@@ -411,13 +437,23 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         }
         pcnt.clear();
         pcnt.shrink_to_fit();
-        if ((r = dev_alloc((void **)&L->delta, sizeof(uint32_t) * cells, "cold pair run offsets"))) break;
+        if ((r = dev_alloc((void **)&L->roff, sizeof(uint32_t) * roff.size(), "cold pair run offsets"))) break;
+        if ((r = dev_alloc((void **)&L->tcell, sizeof(uint32_t) * tcell.size(), "cold tile cells"))) break;
+        if ((r = dev_alloc((void **)&pdelta, sizeof(uint32_t) * cells, "cold cell pair offsets"))) break;
+        if ((r = dev_alloc((void **)&dord, sizeof(uint32_t) * cells, "cold cell ordinals"))) break;
         if ((r = dev_alloc((void **)&edelta, sizeof(uint32_t) * cells, "cold entry run offsets"))) break;
-        if (hipMemcpyAsync(L->delta, pdel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
+        if (hipMemcpyAsync(L->roff, roff.data(), sizeof(uint32_t) * roff.size(), hipMemcpyHostToDevice, g->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(L->tcell, tcell.data(), sizeof(uint32_t) * tcell.size(), hipMemcpyHostToDevice,
+                           g->stream) != hipSuccess ||
+            hipMemcpyAsync(pdelta, pdel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
+                hipSuccess ||
+            hipMemcpyAsync(dord, ord.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
                 hipSuccess ||
             hipMemcpyAsync(edelta, edel.data(), sizeof(uint32_t) * cells, hipMemcpyHostToDevice, g->stream) !=
                 hipSuccess) { r = GPR_E_HIP; break; }
-        if ((r = dev_alloc((void **)&L->wsrc, 4 * npairs, "cold write words"))) break;
+        if ((r = dev_alloc((void **)&L->wsrc, 2 * npairs, "cold write words"))) break;
+        if ((r = dev_alloc((void **)&L->wbase, 4 * (npairs / 64 + 1), "cold write bases"))) break;
         // zero tails: the accumulation reads up to two loop steps past a
         // chunk's end without bounds checks
         if ((r = dev_alloc((void **)&L->cdst, 2 * (ncold + COLD_PAD_E), "cold destinations"))) break;
@@ -428,14 +464,16 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
             break;
         }
         hipLaunchKernelGGL(k_cold_finish, dim3(eg), dim3(256), 0, g->stream, kb.current(), vb.current(), incl, ncold,
-                           nbins, edelta, L->delta, L->wsrc, L->cdst, L->bbase);
+                           edelta, pdelta, dord, L->wsrc, L->wbase, L->cdst, L->bbase);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess) { r = GPR_E_HIP; break; }
         dev_free(k0);
         dev_free(k1);
         dev_free(v0);
         dev_free(v1);
         dev_free(edelta);
-        k0 = k1 = v0 = v1 = edelta = nullptr;
+        dev_free(pdelta);
+        dev_free(dord);
+        k0 = k1 = v0 = v1 = edelta = pdelta = dord = nullptr;
         if ((r = dev_alloc((void **)&L->vals, sizeof(uint32_t) * (npairs + 64), "cold values"))) break;
         L->nwchunks = (uint32_t)(wch.size() / 4);
         L->nachunks = (uint32_t)(ach.size() / 4);
@@ -467,6 +505,8 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     dev_free(v0);
     dev_free(v1);
     dev_free(edelta);
+    dev_free(pdelta);
+    dev_free(dord);
     dev_free(tmp);
     if (r != GPR_OK) {
         if (r == GPR_E_HIP) set_error("PageRank cold-edge layout build failed");
@@ -484,7 +524,7 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
 constexpr int CW_THREADS = 1024;
 constexpr int CW_UNROLL = 16;
 constexpr int CW_STAGE = 8;               // window loads per thread in flight
-constexpr uint32_t CW_DELTA_LDS = 4096;   // run offsets staged in LDS up to this many bins
+constexpr uint32_t CW_DELTA_LDS = 4096;   // a tile's run offsets staged in LDS up to this many cells
 
 // Workgroup i -> chunk: the XCD that runs workgroup i (i mod 8 on this
 // part) takes a contiguous range of chunks, in order.
@@ -494,23 +534,41 @@ __device__ __forceinline__ uint32_t xcd_chunk(uint32_t i, uint32_t count) {
 }
 
 // The copy loop of one write chunk: pairs [p0, p1) of the tile, values
-// from the LDS window, run offsets from dt (LDS or global, fixed per
-// instantiation so that every access has a known address space).
+// from the LDS window, run offsets by cell ordinal - o0 (< nc) from ro (LDS or
+// global, fixed per instantiation so that every access has a known address
+// space).  A wave handles 64 consecutive pairs: the ordinal of a lane's
+// cell is the block's base (the ordinal of its first pair's cell) plus the
+// cell starts among lanes 1..l.  Lanes before p0 read their real words (the
+// previous chunk's pairs, whose starts count); lanes at or past p1 come
+// after every lane in the chunk and read p0's word.
 template <int FMT, class D>
-__device__ __forceinline__ void cold_copy(uint32_t p0, uint32_t p1, const uint32_t *__restrict__ wsrc, D dt,
+__device__ __forceinline__ void cold_copy(uint32_t p0, uint32_t p1, const uint16_t *__restrict__ wsrc,
+                                          const uint32_t *wbase, D ro, uint32_t o0, uint32_t nc,
                                           const typename Codec<FMT>::store_t *win,
                                           typename Codec<FMT>::store_t *__restrict__ vals) {
+    // the block bases are uniform: scalar loads through the constant
+    // address space
+    const __attribute__((address_space(4))) uint32_t *wbs = (const __attribute__((address_space(4))) uint32_t *)wbase;
+    const uint32_t wo = __builtin_amdgcn_readfirstlane(threadIdx.x / 64) * 64;
     // from p0 rounded down to 64: every wave reads whole cache lines
     for (uint32_t base = p0 & ~63u; base < p1; base += CW_THREADS * CW_UNROLL) {
-        uint32_t wv[CW_UNROLL];
+        uint32_t wv[CW_UNROLL], bw[CW_UNROLL];
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
-            wv[u] = __builtin_nontemporal_load(&wsrc[p >= p0 && p < p1 ? p : p0]);
+            wv[u] = __builtin_nontemporal_load(&wsrc[p < p1 ? p : p0]);
+            const uint32_t bs = base + u * CW_THREADS + wo;   // the wave's block
+            bw[u] = wbs[(bs < p1 ? bs : p0) >> 6];
         }
         uint32_t dl[CW_UNROLL];
 #pragma unroll
-        for (int u = 0; u < CW_UNROLL; ++u) dl[u] = dt[wv[u] >> 15];
+        for (int u = 0; u < CW_UNROLL; ++u) {
+            const uint32_t p = base + u * CW_THREADS + threadIdx.x;
+            const uint64_t m = __ballot(wv[u] & 0x8000u);
+            const uint32_t o = bw[u] + mbcnt(m >> 1);
+            const uint32_t i = o - o0;
+            dl[u] = ro[p >= p0 && p < p1 && i < nc ? i : 0u];
+        }
 #pragma unroll
         for (int u = 0; u < CW_UNROLL; ++u) {
             const uint32_t p = base + u * CW_THREADS + threadIdx.x;
@@ -552,7 +610,8 @@ __device__ __forceinline__ double cold_stage(const typename Codec<FMT>::store_t 
 template <int FMT>
 __global__ void __launch_bounds__(CW_THREADS)
 k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
-             uint32_t nbins, const uint32_t *__restrict__ wsrc, const uint32_t *__restrict__ delta,
+             const uint32_t *__restrict__ tcell, const uint16_t *__restrict__ wsrc, const uint32_t *wbase,
+             const uint32_t *__restrict__ roff,
              const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
              const uint32_t *__restrict__ deg, double *__restrict__ mass, const PrScalars *__restrict__ prs) {
     using S = typename Codec<FMT>::store_t;
@@ -568,13 +627,13 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     // x out-degree over its sources (a value is added once per edge)
     const double ms0 = first ? cold_stage<FMT, true>(contrib + lo, deg + lo, w, win)
                              : cold_stage<FMT, false>(contrib + lo, deg + lo, w, win);
-    const uint32_t *dt = delta + (size_t)t * nbins;
-    const bool staged = nbins <= CW_DELTA_LDS;   // the tile's run offsets fit in LDS
+    const uint32_t o0 = tcell[t], nc = tcell[t + 1] - o0;   // the tile's non-empty cells
+    const bool staged = nc <= CW_DELTA_LDS;                  // their run offsets fit in LDS
     if (staged)
-        for (uint32_t i = threadIdx.x; i < nbins; i += CW_THREADS) dls[i] = dt[i];
+        for (uint32_t i = threadIdx.x; i < nc; i += CW_THREADS) dls[i] = roff[o0 + i];
     __syncthreads();
-    if (staged) cold_copy<FMT>(p0, p1, wsrc, (const uint32_t *)dls, win, vals);
-    else cold_copy<FMT>(p0, p1, wsrc, dt, win, vals);
+    if (staged) cold_copy<FMT>(p0, p1, wsrc, wbase, (const uint32_t *)dls, o0, nc, win, vals);
+    else cold_copy<FMT>(p0, p1, wsrc, wbase, roff + o0, o0, nc, win, vals);
     const double ms = block_sum(ms0, red);
     if (threadIdx.x == 0) mass[c] = FMT == GPR_CONTRIB_F16 ? ms / (double)prs->enc_scale : ms;
 }
@@ -852,7 +911,7 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
     ColdLayout *L = g->cold;
 #define CW(F)                                                                                                   \
     hipLaunchKernelGGL(k_cold_write<F>, dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,           \
-                       L->nwchunks, L->tile_lo, L->nbins, L->wsrc, L->delta,                                    \
+                       L->nwchunks, L->tile_lo, L->tcell, L->wsrc, L->wbase, L->roff,                           \
                        (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, g->deg,  \
                        L->mass, g->prs)
     switch (fmt) {
