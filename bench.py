@@ -53,12 +53,13 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 
 # HBM bytes of one PR edgemap (FP32, every edge binned) on relabelled RMAT-26
 # ef16, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the edgemap's
-# kernels (k_cold_write, k_cold_accum, k_cold_combine).  Each counter is
-# scaled by what it reads for k_pr_update<fp32>, a plain stream of 12n bytes
-# in and 8n out: FETCH_SIZE x 2.00, WRITE_SIZE x 1.00.  The summary with the
+# kernels (k_cold_write, k_cold_accum, k_cold_combine_groups,
+# k_cold_combine).  Each counter is scaled by what it reads for
+# k_pr_update<fp32>, a plain stream of 12n bytes in and 8n out: FETCH_SIZE
+# x 2.00, WRITE_SIZE x 1.00.  The summary with the
 # commands is profiles/pr_edgemap_rmat26_pmc.md.  Other configurations
 # report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 13.619e9}
+MEASURED_TRAFFIC = {(26, 16, 0.0): 12.965e9}
 
 
 def pr_edgemap_ms(g):
