@@ -67,6 +67,7 @@ struct Tunables {
     double pr_split_bytes = 0;                // FP32 bytes of pulled sources (0: all binned; >= 4n: no split)
     double fusion_spin_limit = 1 << 22;       // fused push: polls without progress before it gives up
     double fusion_blocks = 0;                 // fused push: cooperative grid size (0: one per CU)
+    double pr_write_stage_cells = 4096;       // PR write: a tile's run offsets go to LDS up to this many cells
 };
 Tunables &tun();
 int num_cus();

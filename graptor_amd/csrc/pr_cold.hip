@@ -611,7 +611,7 @@ template <int FMT>
 __global__ void __launch_bounds__(CW_THREADS)
 k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
              const uint32_t *__restrict__ tcell, const uint16_t *__restrict__ wsrc, const uint32_t *wbase,
-             const uint32_t *__restrict__ roff,
+             const uint32_t *__restrict__ roff, uint32_t stage_max,
              const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
              const uint32_t *__restrict__ deg, double *__restrict__ mass, const PrScalars *__restrict__ prs) {
     using S = typename Codec<FMT>::store_t;
@@ -628,7 +628,7 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     const double ms0 = first ? cold_stage<FMT, true>(contrib + lo, deg + lo, w, win)
                              : cold_stage<FMT, false>(contrib + lo, deg + lo, w, win);
     const uint32_t o0 = tcell[t], nc = tcell[t + 1] - o0;   // the tile's non-empty cells
-    const bool staged = nc <= CW_DELTA_LDS;                  // their run offsets fit in LDS
+    const bool staged = nc <= stage_max;                     // their run offsets fit in LDS (stage_max <= CW_DELTA_LDS)
     if (staged)
         for (uint32_t i = threadIdx.x; i < nc; i += CW_THREADS) dls[i] = roff[o0 + i];
     __syncthreads();
@@ -909,9 +909,10 @@ k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, c
 
 int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
     ColdLayout *L = g->cold;
+    const uint32_t stage_max = (uint32_t)std::min<double>(CW_DELTA_LDS, tun().pr_write_stage_cells);
 #define CW(F)                                                                                                   \
     hipLaunchKernelGGL(k_cold_write<F>, dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,           \
-                       L->nwchunks, L->tile_lo, L->tcell, L->wsrc, L->wbase, L->roff,                           \
+                       L->nwchunks, L->tile_lo, L->tcell, L->wsrc, L->wbase, L->roff, stage_max,                \
                        (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, g->deg,  \
                        L->mass, g->prs)
     switch (fmt) {

@@ -384,7 +384,12 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
  *                       claiming work, or of a reserved queue slot, before
  *                       the step fails with GPR_E_HIP (default 2^22)
  *   fusion_blocks       fused push: blocks of 1024 threads in its
- *                       cooperative grid (0: one per compute unit) */
+ *                       cooperative grid (0: one per compute unit)
+ *   pr_write_stage_cells  PageRank binned edgemap: a source tile's run
+ *                       offsets are staged in LDS when it has at most this
+ *                       many non-empty (tile, bin) cells (capped at 4096,
+ *                       the default), else read from HBM; the results are
+ *                       the same either way */
 int gpr_tune_set(const char *name, double value);
 int gpr_tune_get(const char *name, double *value);
 

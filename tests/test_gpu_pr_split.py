@@ -160,3 +160,28 @@ def test_split_rmat20_chunked_hub_bins(h):
         assert rel_l1(x1, x0) <= 5e-8, rel_l1(x1, x0)
         xb, _ = g.pr_vary([("bf16", 0.0)], max_iter=20)
         assert rel_l1(xb, x64) <= 1e-3, rel_l1(xb, x64)
+
+
+@pytest.mark.parametrize("name", ["rmat14", "er", "star5000"])
+@pytest.mark.parametrize("cells", [0, 2])
+def test_write_run_offsets_from_hbm(name, cells):
+    """A source tile with more non-empty (tile, bin) cells than the write
+    kernel stages in LDS (4096; only above 2^26 vertices) reads its run
+    offsets from HBM.  Capping the staging (tunable pr_write_stage_cells)
+    forces that path for every tile (0) or for the tiles with more than two
+    cells (2): the values land in the same slots, so the ranks are
+    bit-identical to the staged run, FP32 and 16-bit."""
+    index, nbrs = GRAPHS[name]
+    old = G.tune_get("pr_write_stage_cells")
+    try:
+        with G.Graph.from_csr(index, nbrs).reorder() as g:
+            split_at(0)
+            want, _ = g.prv(max_iter=20)
+            wantb, _ = g.pr_vary([("bf16", 0.0)], max_iter=20)
+            G.tune_set("pr_write_stage_cells", cells)
+            x, _ = g.prv(max_iter=20)
+            xb, _ = g.pr_vary([("bf16", 0.0)], max_iter=20)
+        assert np.array_equal(x, want)
+        assert np.array_equal(xb, wantb)
+    finally:
+        G.tune_set("pr_write_stage_cells", old)
