@@ -55,7 +55,8 @@
 // device (edge counts per (tile, bin), a stable radix sort of the edges by
 // (tile, bin) that keeps each source's edges together, pair flags and their
 // prefix sum, pair counts per cell), planned on the host (run offsets,
-// chunks) and kept until the graph's CSR changes.  Entries are the CSR edges
+// the ordinals of the non-empty cells, chunks) and kept until the graph's
+// CSR changes.  Entries are the CSR edges
 // s -> d with s >= from, so directed graphs work the same way.
 #include "internal.hpp"
 #include "device.hpp"
@@ -743,8 +744,7 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     // first entry), so lane l loads vals[pb + l] beside its destination and
     // takes the value of pair pb + r from lane r, r = the pair starts among
     // lanes 1..l (entries past the chunk sit in higher lanes and never
-    // count): no load waits for another.  The next iteration's bases are
-    // loaded one iteration ahead.
+    // count): no load waits for another.
     const uint32_t a0 = p0 & ~63u, lo = p0 - a0, len = p1 - a0;
     const uint32_t *bb = bbase + (a0 >> 6);
     const uint32_t step = nw * 64 * CA_U;
