@@ -26,9 +26,9 @@
 //                   pair, so an entry's pair is the pair of its 64-entry
 //                   block's first entry plus a ballot prefix count; a wave
 //                   loads the block's next 64 pair values beside its
-//                   destinations and hands each lane its value by shuffle.  The sums go into
-//                   LDS as 64-bit fixed-point integers, so the result does
-//                   not depend on the order of the atomics.  A bin in one
+//                   destinations and hands each lane its value by shuffle.
+//                   The sums go into LDS as 64-bit fixed-point integers, so
+//                   the result does not depend on the order of the atomics.  A bin in one
 //                   chunk updates y directly; the chunks of a heavy bin (the
 //                   hub destinations) write integer partials that
 //   k_cold_combine_groups and k_cold_combine add up per destination in two
@@ -40,9 +40,9 @@
 // its degree (RMAT: 0.10 / 0.16 / 0.22 pairs per edge at scales 20 / 22 /
 // 24), so a value crosses HBM once per pair instead of once per edge; the
 // per-edge traffic is the 2 B destination (and per pair a 2 B write word,
-// the 4 B value written and the 4 B value read back).  This is the "partition-centric"
-// form of propagation blocking (Lakhotia, Kannan and Prasanna, USENIX ATC
-// 2018).
+// the 4 B value written and the 4 B value read back).  This is the
+// "partition-centric" form of propagation blocking (Lakhotia, Kannan and
+// Prasanna, USENIX ATC 2018).
 //
 // The fixed-point unit 2^E is chosen per iteration from the total mass M (a
 // deterministic FP64 sum of the tiles' masses): every partial sum is at most
@@ -56,8 +56,8 @@
 // (tile, bin) that keeps each source's edges together, pair flags and their
 // prefix sum, pair counts per cell), planned on the host (run offsets,
 // the ordinals of the non-empty cells, chunks) and kept until the graph's
-// CSR changes.  Entries are the CSR edges
-// s -> d with s >= from, so directed graphs work the same way.
+// CSR changes.  Entries are the CSR edges s -> d with s >= from, so directed
+// graphs work the same way.
 #include "internal.hpp"
 #include "device.hpp"
 #include "pr_codec.hpp"
