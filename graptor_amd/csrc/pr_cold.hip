@@ -749,14 +749,23 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     const uint32_t *bb = bbase + (a0 >> 6);
     const uint32_t step = nw * 64 * CA_U;
     static_assert(3 * nw_max * 64 * CA_U <= COLD_PAD_E, "zero tail shorter than three loop steps");
-    // buffer descriptors from the chunk's first entry and first pair: the
-    // uniform part of each offset goes into the scalar offset.  A block past
-    // the end of the pair-base array reads base 0 (its zero tail); bases are
-    // clamped to the chunk's first pair, so no offset is negative, and the
-    // value descriptor covers exactly the pairs from there on (+ the tail)
-    const __amdgpu_buffer_rsrc_t krs = buf_rsrc(cdst + a0, 2u * (len + 2 * step));
+    // Buffer descriptors from the chunk's first entry and first pair, each
+    // covering what the chunk's blocks use: the destinations up to the
+    // chunk's end rounded up to a block, the pairs up to the window of the
+    // first block past that end (all pairs from the first on, with their
+    // tail, when that base is the zero tail).  The whole offset of a load is
+    // its vector offset, which the range check covers, so the loads of the
+    // step prefetched past the chunk's end return 0 without reading memory.
+    // A block past the end of the pair-base array reads base 0 (its zero
+    // tail); bases are clamped to the chunk's first pair, so no offset is
+    // negative.
+    const uint32_t lenr = (len + 63u) & ~63u;
+    const __amdgpu_buffer_rsrc_t krs = buf_rsrc(cdst + a0, 2u * lenr);
     const uint32_t pfirst = __builtin_amdgcn_readfirstlane(bb[0]);
-    const uint64_t vbytes = (npairs + 64 - pfirst) * (uint64_t)sizeof(S);
+    const uint32_t pend = __builtin_amdgcn_readfirstlane(bb[lenr >> 6]);
+    const uint64_t vall = npairs + 64 - pfirst;
+    const uint64_t vcnt = pend > pfirst && (uint64_t)(pend - pfirst) + 64 < vall ? (uint64_t)(pend - pfirst) + 64 : vall;
+    const uint64_t vbytes = vcnt * (uint64_t)sizeof(S);
     const __amdgpu_buffer_rsrc_t vrs = buf_rsrc(vals + pfirst, vbytes < 0xffffffffull ? (uint32_t)vbytes : 0xffffffffu);
     // Two steps in flight: the loads of the next step are issued before
     // the shuffles and atomics of this one, so that a wave's LDS work
@@ -766,18 +775,16 @@ k_cold_accum(const uint32_t *__restrict__ achunk, uint32_t n, const uint16_t *cd
     auto issue = [&](uint32_t at, uint32_t *kk, uint32_t *vl) {
 #pragma unroll
         for (int u = 0; u < CA_U; ++u) {
-            kk[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(krs, (int)(2 * ln), (int)(2 * (at + u * nw * 64)),
-                                                                   BUF_NT);
+            kk[u] = (uint16_t)__builtin_amdgcn_raw_buffer_load_b16(krs, (int)(2 * (ln + at + u * nw * 64)), 0, BUF_NT);
             // the base was loaded a step ago and is uniform by construction;
             // readfirstlane puts it, and the offset derived from it, in
-            // scalar registers, as the load's scalar offset must be (an
-            // offset left in a vector register, as a mixed signed/unsigned
-            // max() or a fused clamp-subtract produce, becomes a waterfall
-            // loop around the load)
+            // scalar registers (an offset left in a vector register, as a
+            // mixed signed/unsigned max() or a fused clamp-subtract produce,
+            // makes the compiler treat it as non-uniform)
             const uint32_t pbr = (uint32_t)__builtin_amdgcn_readfirstlane(pbn[u]);
             const uint32_t pb = pbr > pfirst ? pbr : pfirst;
             const uint32_t soff = (uint32_t)__builtin_amdgcn_readfirstlane((int)(sizeof(S) * (pb - pfirst)));
-            vl[u] = buf_load(vrs, sizeof(S) * ln, soff, S());
+            vl[u] = buf_load(vrs, sizeof(S) * ln + soff, 0u, S());
         }
     };
     // the bases are uniform: read through the constant address space they
