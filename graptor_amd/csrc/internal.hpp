@@ -114,7 +114,7 @@ struct PrScalars {
 constexpr uint32_t COLD_BIN = 16384;
 constexpr uint32_t COLD_TILE_V = 32768;
 constexpr uint64_t COLD_TILE_E = 1ull << 24;
-constexpr uint64_t COLD_WCHUNK = 262144;   // pairs per write chunk
+constexpr uint64_t COLD_WCHUNK = 131072;   // pairs per write chunk
 constexpr uint64_t COLD_CHUNK_E = 1u << 20;
 constexpr uint64_t COLD_PAD_E = 65536;          // zero tail of the destination array (and of the pair bases / 64)
 constexpr uint32_t COLD_DIRECT = 0xffffffffu;   // chunk slot of a bin in one chunk
