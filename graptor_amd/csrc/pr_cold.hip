@@ -640,7 +640,7 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
 }
 
 constexpr int CA_THREADS = 1024;
-constexpr int CA_U = 16;                         // 64-entry blocks of one wave in flight
+constexpr int CA_U = 8;                          // 64-entry blocks of one wave in flight
 constexpr uint32_t nw_max = CA_THREADS / 64;
 
 // v * 2^-E truncated toward zero, as an integer (sh = -E - 32).  The unit
