@@ -59,7 +59,7 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 # x 2.00, WRITE_SIZE x 1.00.  The summary with the
 # commands is profiles/pr_edgemap_rmat26_pmc.md.  Other configurations
 # report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 12.678e9}
+MEASURED_TRAFFIC = {(26, 16, 0.0): 12.385e9}
 
 
 def pr_edgemap_ms(g):
