@@ -62,6 +62,30 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 MEASURED_TRAFFIC = {(26, 16, 0.0): 12.385e9}
 
 
+def run_bytes(trace, n, algo):
+    """Algorithmic HBM bytes of one BFSLVLv or CCv run from its per-step
+    trace: sum over steps of 8*V_scan + 4*E_scan + w*E_scan + 4*V_upd.
+    Push: the frontier's vertices (offsets) and out-edges (neighbour ID plus
+    the destination's 4 B value), w = 4.  Pull: the destinations that pass
+    the filter (BFS: the unvisited ones; CC: every vertex) and the in-edges
+    scanned up to early exit; w = 1/8 for BFS (a frontier-bit test) and 4
+    for CC (a label gather).  A fused step counts its relaxations as edges.
+    V_upd: the vertices the step activates (4 B each)."""
+    total = 0.0
+    reached = 1
+    for st in trace:
+        e = st["edges_scanned"]
+        if st["direction"] == 2:   # pull
+            v = (n - reached) if algo == "bfs" else n
+            w = 0.125 if algo == "bfs" else 4.0
+        else:
+            v = st["nactv_in"]
+            w = 4.0
+        total += 8.0 * v + (4.0 + w) * e + 4.0 * st["nactv_out"]
+        reached += st["nactv_out"]
+    return total
+
+
 def pr_edgemap_ms(g):
     """Per-scope average ms of the profiled PR edgemaps and their sum per
     edgemap (each scope runs at most once per edgemap)."""
@@ -96,10 +120,13 @@ def run(args):
     # the timed runs leave their results on the device, as Graptor's driver
     # times run() alone (driver.C:245-255); one untimed run afterwards
     # downloads them for the parity check
+    algos = set(args.algos.split(","))
+    none = G.RunInfo()
+
     def step(download=False):
-        lv, ib = g.bfslvl(src, download=download)
-        lab, ic = g.ccv(download=download)
-        x, ip = g.prv(max_iter=args.pr_iters, tol=0.0, download=download)
+        lv, ib = g.bfslvl(src, download=download) if "bfs" in algos else (None, none)
+        lab, ic = g.ccv(download=download) if "cc" in algos else (None, none)
+        x, ip = g.prv(max_iter=args.pr_iters, tol=0.0, download=download) if "pr" in algos else (None, none)
         return (lv, ib), (lab, ic), (x, ip)
 
     for w in range(args.warmup):
@@ -124,6 +151,18 @@ def run(args):
     avg_pull_ms, per_scope, pull_launches = pr_edgemap_ms(g)
     g.profile(False)
     (lv, _), (lab, _), (x, _) = step(download=True)
+    if algos != {"bfs", "cc", "pr"}:
+        # a subset of the step (for counter runs): timings only
+        print(json.dumps({"algos": sorted(algos), "device_ms": {"bfs": tb / args.steps, "cc": tc / args.steps,
+                                                                 "pr": tp / args.steps}}), flush=True)
+        g.destroy()
+        return None
+    # the BFS and CC runs once more with per-step traces, for their
+    # algorithmic bytes
+    g.bfslvl(src, download=False, trace=True)
+    b_bfs = run_bytes(g.trace(), n, "bfs")
+    g.ccv(download=False, trace=True)
+    b_cc = run_bytes(g.trace(), n, "cc")
     total_edges = eb + ec + ep
     value = total_edges / wall / 1e9
     bytes_pull = pr_pull_bytes(n, m, 4)
@@ -145,7 +184,7 @@ def run(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32 (BFS levels, CC labels); f32 store / f64 accumulate (PR)",
+        "dtype": "u32 (BFS levels, CC labels); f32 store / 64-bit fixed-point accumulate (PR)",
         "data": "synthetic RMAT generated on the GPU from a fixed seed",
         "config": {
             "workload": "RMAT-%d ef%d a=.57 b=.19 c=.19 d=.05 symmetric, cleaned: BFSLVLv from the highest-degree "
@@ -172,6 +211,14 @@ def run(args):
             "launches": pull_launches,
             "pr_split_bytes": split,
         },
+        # BFSLVLv and CCv against the same peak, on their algorithmic bytes
+        # per run (run_bytes) over the run's device time
+        "roofline_bfs": {"bound": "hbm", "achieved": b_bfs / (tb / args.steps * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": b_bfs / (tb / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_run": b_bfs, "avg_run_ms": tb / args.steps},
+        "roofline_cc": {"bound": "hbm", "achieved": b_cc / (tc / args.steps * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": b_cc / (tc / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_run": b_cc, "avg_run_ms": tc / args.steps},
     }
     # reduced-precision PR, per iteration (PR_vary contribution formats)
     if not args.no_precision:
@@ -244,21 +291,27 @@ def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu, apps=None, outs=None):
     log("downloaded graph for the CPU baseline (%.1fs)" % (time.time() - t0))
     n, m = len(index) - 1, len(nbrs)
     src_int = g.remap_id(src)
-    t = time.perf_counter()
-    lv, bi = O.graptor_bfs(index, nbrs, src_int, threads)
-    t_bfs = time.perf_counter() - t
+    rounds = args.cpu_rounds
+
+    def median_time(f):
+        """Median wall time of `rounds` runs (driver.C:195 -rounds) and the
+        last run's result."""
+        ts, out = [], None
+        for _ in range(rounds):
+            t = time.perf_counter()
+            out = f()
+            ts.append(time.perf_counter() - t)
+        return float(np.median(ts)), out
+
+    t_bfs, (lv, bi) = median_time(lambda: O.graptor_bfs(index, nbrs, src_int, threads))
     deg = np.diff(index)
     e_bfs = int(deg[lv <= n].sum())
-    log("cpu bfs %.2fs" % t_bfs)
-    t = time.perf_counter()
-    lab, ci = O.graptor_cc(index, nbrs, threads)
-    t_cc = time.perf_counter() - t
-    log("cpu cc %.2fs" % t_cc)
+    log("cpu bfs %.2fs (median of %d)" % (t_bfs, rounds))
+    t_cc, (lab, ci) = median_time(lambda: O.graptor_cc(index, nbrs, threads))
+    log("cpu cc %.2fs (median of %d)" % (t_cc, rounds))
     pr_it = args.cpu_pr_iters
-    t = time.perf_counter()
-    xc, it, _ = O.graptor_pr(index, nbrs, pr_it, 0.0, threads)
-    t_pr = time.perf_counter() - t
-    log("cpu pr %d iters %.2fs" % (pr_it, t_pr))
+    t_pr, (xc, it, _) = median_time(lambda: O.graptor_pr(index, nbrs, pr_it, 0.0, threads))
+    log("cpu pr %d iters %.2fs (median of %d)" % (pr_it, t_pr, rounds))
     tot_e = e_bfs + m + pr_it * m
     tot_t = t_bfs + t_cc + t_pr
     base = {
@@ -266,8 +319,9 @@ def cpu_baseline(g, args, src, lv_gpu, lab_gpu, x_gpu, apps=None, outs=None):
         "unit": "GTEPS",
         "cores": threads,
         "kind": "port",
-        "sample": "one BFSLVLv run + one CCv run + %d PRv iterations on the same RMAT-%d graph "
-                  "(OpenMP restatement of Graptor's push/pull engine, oracle/oracle.c)" % (pr_it, args.scale),
+        "sample": "one BFSLVLv run + one CCv run + %d PRv iterations on the same RMAT-%d graph, each the median "
+                  "of %d runs (OpenMP restatement of Graptor's push/pull engine, oracle/oracle.c)"
+                  % (pr_it, args.scale, rounds),
         "gteps": {"bfs": e_bfs / t_bfs / 1e9, "cc": m / t_cc / 1e9, "pr_iter_fp32": pr_it * m / t_pr / 1e9},
         "seconds": {"bfs": t_bfs, "cc": t_cc, "pr": t_pr},
         "cpu": _cpu_model(),
@@ -319,7 +373,10 @@ def main():
     ap.add_argument("--scale", type=int, default=26)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--pr-iters", type=int, default=20)
-    ap.add_argument("--cpu-pr-iters", type=int, default=2)
+    ap.add_argument("--cpu-pr-iters", type=int, default=5)
+    ap.add_argument("--cpu-rounds", type=int, default=5, help="CPU baseline: median of this many runs")
+    ap.add_argument("--algos", default="bfs,cc,pr",
+                    help="run a subset of the step (for counter runs; prints timings only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-reorder", action="store_true")
@@ -332,7 +389,8 @@ def main():
               "documentation); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     res = run(args)
-    print(json.dumps(res), flush=True)
+    if res is not None:
+        print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

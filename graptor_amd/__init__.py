@@ -20,10 +20,10 @@ FRONTIER_EMPTY, FRONTIER_ALL, FRONTIER_LIST, FRONTIER_BITMAP = range(4)
 RELAX_MIN_PLUS1, RELAX_MIN, RELAX_PR_SUM, RELAX_SETIF = range(4)
 UNSET = 0xFFFFFFFF   # an unset BFSv parent
 DST_NONE, DST_GT, DST_NE = range(3)
-RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE = range(3)
+RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE, RECORD_REDUCTION_OR_METHOD = range(4)
 DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
 CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T, CONTRIB_SCFP16 = range(6)
-VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER = range(4)
+VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER, VMAP_COPY_FRONTIER = range(5)
 CONTRIB_NAMES = {"fp32": CONTRIB_F32, "bf16": CONTRIB_BF16, "fp16": CONTRIB_F16, "cfp16": CONTRIB_CFP16,
                  "bf16t": CONTRIB_BF16T, "scfp16": CONTRIB_SCFP16}
 # APR_vary's default variant 2046: FP32 contributions down to 1e-1, then
@@ -55,7 +55,8 @@ class IterStats(C.Structure):
 
 class RunOpts(C.Structure):
     _fields_ = [("direction", C.c_int), ("out_kind", C.c_int), ("threshold_pct", C.c_float),
-                ("max_iter", C.c_int), ("tol", C.c_double), ("trace", C.c_int), ("fusion", C.c_int)]
+                ("max_iter", C.c_int), ("tol", C.c_double), ("trace", C.c_int), ("fusion", C.c_int),
+                ("deferred_update", C.c_int)]
 
 
 class RunInfo(C.Structure):
@@ -202,13 +203,18 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-FUSION_OFF, FUSION_ON, FUSION_DEFAULT = range(3)
+FUSION_OFF, FUSION_ON, FUSION_DEFAULT, FUSION_SMALL = range(4)
 
 
-def opts(direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, max_iter=100, tol=1e-7, trace=False, fusion=0):
+def opts(direction=DIR_AUTO, out_kind=0, threshold_pct=0.0, max_iter=100, tol=1e-7, trace=False, fusion=0,
+         deferred_update=0):
     """Run options.  fusion: 0 = the benchmark's reference default, -1 = off,
-    1 = Graptor's FUSION 1 rule, 2 = every sparse step fused."""
-    return RunOpts(direction, out_kind, threshold_pct, max_iter, tol, 1 if trace else 0, fusion)
+    1 = Graptor's FUSION 1 rule, 2 = every sparse step fused, 3 = small
+    levels fused (GPR_FUSION_SMALL).
+    deferred_update: 1 = the benchmarks' DEFERRED_UPDATE=1 record forms
+    (method / reduction_or_method with a maintained previous copy)."""
+    return RunOpts(direction, out_kind, threshold_pct, max_iter, tol, 1 if trace else 0, fusion,
+                   1 if deferred_update else 0)
 
 
 class Graph:
@@ -408,7 +414,7 @@ class Graph:
         return Frontier(self)
 
     def edgemap(self, relax, prop, frontier_in=None, frontier_out=None, dst_filter=DST_NONE, dst_arg=0,
-                record=RECORD_REDUCTION, direction=DIR_AUTO, out_kind=0, threshold_pct=5.0, prev=None,
+                record=RECORD_REDUCTION, direction=DIR_AUTO, out_kind=0, threshold_pct=0.0, prev=None,
                 contrib=None, contrib_fmt=CONTRIB_F32, damping=0.85, fusion=FUSION_OFF):
         d = EmapDesc(relax, prop.h, prev.h if prev else None, 0, dst_filter, dst_arg, record, direction, out_kind,
                      threshold_pct, contrib.h if contrib else None, contrib_fmt, damping, fusion)

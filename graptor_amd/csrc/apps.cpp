@@ -9,11 +9,16 @@
 //   APR_vary  bench/APR_vary.C:378-1074 (run_prec :378-640, run :900-1074)
 //   KC_bucket bench/KC_bucket.C:104-330  (run; validate :332-460)
 //
-// Each benchmark runs the reference's DEFERRED_UPDATE=0 configuration of
-// the record operator (api::record(output, api::reduction, api::strong),
-// BFSLVLv.C:173 and CCv.C:543-549): a vertex joins the next frontier when
-// its relax changed it.  For BFS levels and min-label CC this activates the
-// same vertices as the changed-vs-previous method and needs no prev copy.
+// By default each benchmark runs the reference's DEFERRED_UPDATE=0
+// configuration of the record operator (api::record(output, api::reduction,
+// api::strong), BFSLVLv.C:173 and CCv.C:543-549): a vertex joins the next
+// frontier when its relax changed it.  With gpr_run_opts::deferred_update
+// set they run the reference's default DEFERRED_UPDATE=1 instead: BFSLVLv
+// records the method level[d] != prev_level[d] and maintains prev_level
+// over each output (BFSLVLv.C:166-172, 218-220); CCv records
+// reduction_or_method with IDs[d] != prevIDs[d] and copies prevIDs = IDs
+// after each step (CCv.C:536-549, 578-580).  For BFS levels and min-label
+// CC both configurations activate the same vertices.
 #include "../../include/gpr/api.hpp"
 #include "apr.hpp"
 #include "kcore.hpp"
@@ -26,6 +31,7 @@
 
 struct gpr_graph;
 namespace gpr {
+bool threshold_sparse(float pct, uint64_t nactv, uint64_t nacte, uint64_t m);
 std::vector<gpr_iter_stats> &trace_of(gpr_graph *g);
 uint32_t graph_max_degree(const gpr_graph *g);
 int ids_download(gpr_graph *g, const gpr_prop *p, uint32_t *host);
@@ -39,14 +45,16 @@ namespace A = gpr::api;
 namespace {
 
 gpr_run_opts defaults(const gpr_run_opts *o) {
+    // threshold_pct 0: each benchmark's own rule (BFSLVLv and BFSv:
+    // default_threshold, integer m/20; CCv: frac_threshold(5))
     gpr_run_opts d = {};
     d.direction = GPR_DIR_AUTO;
-    d.threshold_pct = 5.f;
+    d.threshold_pct = 0.f;
     d.max_iter = 100;
     d.tol = 1e-7;
     if (o) {
         d = *o;
-        if (d.threshold_pct <= 0.f) d.threshold_pct = 5.f;
+        if (!(d.threshold_pct > 0.f)) d.threshold_pct = 0.f;
         if (d.max_iter <= 0) d.max_iter = 100;
     }
     return d;
@@ -74,17 +82,28 @@ template <class F> int guarded(F &&f) {
 struct BFSLVLv {
     Graph GA;
     gpr_run_opts o;
-    A::vertexprop<uint32_t> a_level;
+    A::vertexprop<uint32_t> a_level, a_prev_level;
     int iter = 0;
-    BFSLVLv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), a_level(GA, "current level") {}
-    ~BFSLVLv() { a_level.del(); }
+    BFSLVLv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), a_level(GA, "current level") {
+        if (o.deferred_update) a_prev_level = A::vertexprop<uint32_t>(GA, "previous level");
+    }
+    ~BFSLVLv() {
+        a_level.del();
+        a_prev_level.del();
+    }
 
     void run(uint32_t start, gpr_run_info *info) {
         const uint32_t n = GA.numVertices();
+        const bool deferred = o.deferred_update != 0;
         // BFSLVLv.C:117-129: ilvl = n+1 everywhere, 0 at the start vertex
+        // (the previous level too under DEFERRED_UPDATE)
         A::make_lazy_executor(GA).vertex_map_fill(a_level, n + 1).materialize();
         A::frontier F = A::frontier::create(GA, start);
         A::make_lazy_executor(GA).vertex_map_fill(F, a_level, 0).materialize();
+        if (deferred) {
+            A::make_lazy_executor(GA).vertex_map_fill(a_prev_level, n + 1).materialize();
+            A::make_lazy_executor(GA).vertex_map_fill(F, a_prev_level, 0).materialize();
+        }
         check(gpr_policy_reset(GA.get()), "policy reset");
         info->edges += F.nActiveEdges();
         iter = 0;
@@ -93,25 +112,40 @@ struct BFSLVLv {
         // 2 = always
         const int fmode = o.fusion == 0 ? -1 : o.fusion;
         bool enable_fusion = fmode == 2 || (fmode == 1 && (uint64_t)graph_max_degree(GA.get()) * (128 * 1024) < n);
+        // 3: small-level fusion, decided per step by the engine
+        const int small = fmode == 3 ? GPR_FUSION_SMALL : -1;
         bool fused = false;
         while (!F.isEmpty()) {
             A::frontier output;
+            // DEFERRED_UPDATE: api::record(output, level[d] != prev_level[d],
+            // strong), BFSLVLv.C:166-172; else the reduction, :173
+            const A::record_arg rec = deferred ? A::record(output, A::differs(a_level, a_prev_level), A::strong)
+                                               : A::record(output, A::reduction, A::strong);
             auto lazy = A::edgemap(GA,
-                                   A::record(output, A::reduction, A::strong),
+                                   rec,
                                    A::filter(A::weak, A::src, F),
                                    A::filter(A::weak, A::dst, A::gt(a_level, (uint32_t)iter + 1)),
                                    A::relax(A::min_plus1(a_level)),
                                    A::config(o.direction, o.threshold_pct, o.out_kind),
-                                   A::fusion_select(enable_fusion));
+                                   small >= 0 ? A::fusion_mode(small) : A::fusion_select(enable_fusion));
             lazy.materialize();
             record_step(GA.get(), o, lazy.stats(), info);
             fused |= lazy.stats().fused != 0;
+            if (deferred) {
+                // maintain_copies(part, output, a_prev_level, a_level),
+                // BFSLVLv.C:218-220.  A fused step changed levels that its
+                // (empty or handed-back) output does not list: copy them all.
+                if (lazy.stats().fused)
+                    A::make_lazy_executor(GA).vertex_map_copy(a_prev_level, a_level).materialize();
+                else
+                    A::maintain_copies(GA, output, a_prev_level, a_level);
+            }
             info->edges += output.nActiveEdges();
             F.del();
             F = output;
             ++iter;
             // api.h:150-153 default_threshold::is_sparse on the new frontier
-            if (fmode == 1 && (float)((uint64_t)F.nActiveVertices() + F.nActiveEdges()) > (float)GA.numEdges() / 20.f)
+            if (fmode == 1 && !threshold_sparse(0.f, F.nActiveVertices(), F.nActiveEdges(), GA.numEdges()))
                 enable_fusion = true;
         }
         F.del();
@@ -175,28 +209,40 @@ struct BFSv {
 struct CCv {
     Graph GA;
     gpr_run_opts o;
-    A::vertexprop<uint32_t> m_IDs;
+    A::vertexprop<uint32_t> m_IDs, m_prevIDs;
     int iter = 0;
-    CCv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), m_IDs(GA, "IDs") {}
-    ~CCv() { m_IDs.del(); }
+    CCv(gpr_graph *g, const gpr_run_opts &opts) : GA(g), o(opts), m_IDs(GA, "IDs") {
+        if (o.deferred_update) m_prevIDs = A::vertexprop<uint32_t>(GA, "previous IDs");
+    }
+    ~CCv() {
+        m_IDs.del();
+        m_prevIDs.del();
+    }
 
     // CCv.C:520-588; FUSION (CCv.C:47-54, 562-566): 1 by default, i.e.
     // api::fusion with default_fusion_select; 2 fuses every sparse step
     int fusion_mode() const {
         const int f = o.fusion == 0 ? 1 : o.fusion;
-        return f == 2 ? GPR_FUSION_ON : f == 1 ? GPR_FUSION_DEFAULT : GPR_FUSION_OFF;
+        return f == 3 ? GPR_FUSION_SMALL : f == 2 ? GPR_FUSION_ON : f == 1 ? GPR_FUSION_DEFAULT : GPR_FUSION_OFF;
     }
     A::frontier emap_step(A::frontier &F, float threshold, gpr_run_info *info) {
         A::frontier output;
         int dir = o.direction;
         float pct = threshold;
+        // DEFERRED_UPDATE: api::record(output, api::reduction_or_method,
+        // IDs[d] != prevIDs[d], strong), CCv.C:536-547; else the reduction
+        const A::record_arg rec = o.deferred_update
+                                      ? A::record(output, A::reduction_or_method, A::differs(m_IDs, m_prevIDs), A::strong)
+                                      : A::record(output, A::reduction, A::strong);
         auto lazy = A::edgemap(GA,
                                A::config(dir, pct, o.out_kind),
-                               A::record(output, A::reduction, A::strong),
+                               rec,
                                A::filter(A::weak, A::src, F),
                                A::filter(A::weak, A::dst, A::ne(m_IDs, 0)),
                                A::relax(A::min_label(m_IDs)),
                                A::fusion_mode(fusion_mode()));
+        // .vertex_map(prevIDs[v] = IDs[v]) chained to the edgemap, CCv.C:578-580
+        if (o.deferred_update) lazy.vertex_map_copy(m_prevIDs, m_IDs);
         lazy.materialize();
         record_step(GA.get(), o, lazy.stats(), info);
         return output;
@@ -232,11 +278,15 @@ struct CCv {
             G0.del();
             ++iter;
         }
+        // the reference leaves prevIDs unset until the first step (a pull,
+        // which records the reduction); here it starts as a copy of IDs
+        if (o.deferred_update) A::make_lazy_executor(GA).vertex_map_copy(m_prevIDs, m_IDs).materialize();
         bool switched = false, have_checked = false;
         A::frontier zeros;
         A::frontier F = A::frontier::all_true(GA);
         while (!F.isEmpty()) {
-            A::frontier output = emap_step(F, o.threshold_pct, info);
+            // emap_step(F, 5), CCv.C:309: frac_threshold(5)
+            A::frontier output = emap_step(F, o.threshold_pct > 0.f ? o.threshold_pct : 5.f, info);
             // PUSH_ZERO, CCv.C:373-388: after one step, if few vertices carry
             // label 0, push the zero label first and the rest afterwards
             if (!have_checked) {

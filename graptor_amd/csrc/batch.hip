@@ -118,9 +118,13 @@ __device__ __forceinline__ void uf_link(const PT &P, uint32_t u, uint32_t v) {
     }
 }
 
+// Edge offsets are clamped to [0, m] and edges that leave the graph are
+// skipped, so a malformed batch in caller memory (gpr_batch_cc_noalloc,
+// which cannot check it on the host) yields wrong labels, never an access
+// outside the arrays.
 template <class PT>
 __device__ void solve_graph(const PT &P, uint32_t base, uint32_t nv, const uint64_t *__restrict__ index,
-                            const uint32_t *__restrict__ nbrs, uint32_t *labels) {
+                            const uint32_t *__restrict__ nbrs, uint64_t m, uint32_t *labels) {
     for (uint32_t v = threadIdx.x; v < nv; v += BCC_THREADS) P.set(v, v);
     __syncthreads();
     const unsigned ln = lane();
@@ -130,14 +134,14 @@ __device__ void solve_graph(const PT &P, uint32_t base, uint32_t nv, const uint6
         const uint32_t v = v0 + ln;
         uint64_t b = 0, e = 0;
         if (v < nv) {
-            b = index[base + v];
-            e = index[base + v + 1];
+            b = min(index[base + v], m);
+            e = min(max(index[base + v + 1], b), m);
         }
         const bool heavy = e - b > 32;
         if (!heavy)
             for (uint64_t i = b; i < e; ++i) {
                 const uint32_t u = nbrs[i] - base;
-                if (u > v) uf_link(P, v, u);
+                if (u > v && u < nv) uf_link(P, v, u);
             }
         uint64_t hm = __ballot(heavy);
         while (hm) {
@@ -147,7 +151,7 @@ __device__ void solve_graph(const PT &P, uint32_t base, uint32_t nv, const uint6
             const uint32_t hv = v0 + L;
             for (uint64_t i = hb + ln; i < he; i += 64) {
                 const uint32_t u = nbrs[i] - base;
-                if (u > hv) uf_link(P, hv, u);
+                if (u > hv && u < nv) uf_link(P, hv, u);
             }
         }
     }
@@ -160,27 +164,31 @@ __global__ void k_batch_reset(unsigned int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
+// work: queue order of the graphs (null: index order)
 __global__ void __launch_bounds__(BCC_THREADS)
-k_batch_cc(uint32_t count, const uint32_t *__restrict__ work, const uint32_t *__restrict__ voff,
+k_batch_cc(uint32_t count, uint32_t n, const uint32_t *__restrict__ work, const uint32_t *__restrict__ voff,
            const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs, uint32_t *labels,
            uint32_t *gparent, unsigned int *queue) {
     __shared__ uint32_t words[BCC_LDS_V / 2];
     __shared__ uint32_t s_item;
+    const uint64_t m = index[n];
     while (true) {
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1u);
         __syncthreads();
         const uint32_t item = s_item;
         __syncthreads();
         if (item >= count) return;    // every workgroup reaches this exit
-        const uint32_t gidx = work[item];
-        const uint32_t base = voff[gidx], nv = voff[gidx + 1] - base;
+        const uint32_t gidx = work ? work[item] : item;
+        const uint32_t base = voff[gidx], end = voff[gidx + 1];
+        if (end < base || end > n) continue;   // malformed offsets: the graph is skipped
+        const uint32_t nv = end - base;
         if (nv <= BCC_LDS_V) {
             Parents<uint16_t> P{words};
-            solve_graph(P, base, nv, index, nbrs, labels);
+            solve_graph(P, base, nv, index, nbrs, m, labels);
         } else {
             // graphs are vertex-disjoint, so each owns its slice of gparent
             Parents<uint32_t> P{gparent + base};
-            solve_graph(P, base, nv, index, nbrs, labels);
+            solve_graph(P, base, nv, index, nbrs, m, labels);
         }
     }
 }
@@ -466,7 +474,7 @@ extern "C" int gpr_batch_cc(gpr_batch *b, uint32_t *labels, gpr_run_info *info) 
         const unsigned grid = (unsigned)std::max(1, std::min<int>(cus, (int)b->count));
         GPR_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
         hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->queue);
-        hipLaunchKernelGGL(k_batch_cc, dim3(grid), dim3(BCC_THREADS), 0, b->stream, b->count, b->work, b->voff,
+        hipLaunchKernelGGL(k_batch_cc, dim3(grid), dim3(BCC_THREADS), 0, b->stream, b->count, b->n, b->work, b->voff,
                            b->index, b->nbrs, b->labels, b->gparent, b->queue);
         hipError_t ce = hipStreamEndCapture(b->stream, &b->graph);
         if (ce != hipSuccess) {

@@ -124,12 +124,26 @@ __device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounter
 constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ITEMS;
 constexpr int PUSH_LDS = PUSH_TILE + 1;
 
+// Push activation of a destination whose relax just changed it
+// (csr_sparse_with_f, dsl/edgemap.h:1497-1507: relax && update): the
+// reduction activates it; the method form (and reduction_or_method, which
+// pushes with the method) only when p[d], read right after the relax,
+// differs from prev[d].
+__device__ __forceinline__ bool push_records(int record, const uint32_t *prop, const uint32_t *prev, uint32_t d) {
+    if (record == GPR_RECORD_REDUCTION) return true;
+    const uint32_t now = __hip_atomic_load(&prop[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return now != prev[d];
+}
+
+// bits == nullptr: the step records nothing (no record argument, or no
+// output frontier); the relaxes still run.
 template <int RELAX>
 __global__ void __launch_bounds__(PUSH_THREADS)
 k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restrict__ pos,
        const uint64_t *__restrict__ btot, uint64_t E, const uint64_t *__restrict__ index,
        const uint32_t *__restrict__ nbrs, const uint32_t *__restrict__ deg, uint32_t *prop, int dkind,
-       uint32_t darg, uint64_t *bits, uint32_t *out_list, DevCounters *ctr, PolicyCtx pol) {
+       uint32_t darg, int record, const uint32_t *__restrict__ prev, uint64_t *bits, uint32_t *out_list,
+       DevCounters *ctr, PolicyCtx pol) {
     __shared__ uint64_t s_pos[PUSH_LDS];
     __shared__ uint32_t s_v[PUSH_LDS];
     __shared__ uint32_t q[PUSH_TILE];
@@ -184,7 +198,8 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
                 if constexpr (RELAX == GPR_RELAX_SETIF) {
                     // single trigger: the CAS that replaces the open value
                     // wins, so each destination is set and activated once
-                    if ((pd & SETIF_OPEN) && dst_ok(pd, dkind, darg) && atomicCAS(&prop[d], pd, s) == pd) {
+                    if ((pd & SETIF_OPEN) && dst_ok(pd, dkind, darg) && atomicCAS(&prop[d], pd, s) == pd &&
+                        bits && push_records(record, prop, prev, d)) {
                         const uint64_t bit = 1ull << (d & 63);
                         const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
                         act = !(ow & bit);
@@ -193,7 +208,7 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
                     const uint32_t nv = relax_val<RELAX>(prop[s]);
                     if (nv < pd) {
                         const uint32_t old = atomicMin(&prop[d], nv);
-                        if (old > nv) {
+                        if (old > nv && bits && push_records(record, prop, prev, d)) {
                             const uint64_t bit = 1ull << (d & 63);
                             const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
                             act = !(ow & bit);
@@ -325,7 +340,9 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         }
         const bool changed = valid && cur < orig;
         if (changed) prop[d] = cur;
-        const bool rec = record == GPR_RECORD_REDUCTION ? changed
+        // a pull privatises its destinations, so reduction_or_method
+        // records the reduction (api.h:1401-1416)
+        const bool rec = record == GPR_RECORD_REDUCTION || record == GPR_RECORD_REDUCTION_OR_METHOD ? changed
                          : record == GPR_RECORD_METHOD_NE ? (valid && cur != prev[d])
                                                           : false;
         const uint64_t word = __ballot(rec);
@@ -366,6 +383,18 @@ __global__ void k_vmap_fill_list32(uint32_t *p, const uint32_t *__restrict__ lis
 __global__ void k_vmap_fill_bits32(uint32_t *p, const uint64_t *__restrict__ bits, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         if (bit_test(bits, i)) p[i] = v;
+}
+__global__ void k_vmap_copy_list32(uint32_t *p, const uint32_t *__restrict__ q, const uint32_t *__restrict__ list,
+                                   uint32_t k) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x) {
+        const uint32_t v = list[i];
+        p[v] = q[v];
+    }
+}
+__global__ void k_vmap_copy_bits32(uint32_t *p, const uint32_t *__restrict__ q, const uint64_t *__restrict__ bits,
+                                   uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (bit_test(bits, i)) p[i] = q[i];
 }
 
 // vertex_filter over a bitmap or all-true frontier: out bit v = in(v) &&
@@ -447,14 +476,32 @@ int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *o
 PolicyCtx policy_ctx(gpr_graph *g, uint64_t nacte_in);
 void policy_after_step(gpr_graph *g, uint64_t nacte_in, uint32_t decision);
 
+// The push/pull switch.  pct <= 0: default_threshold::is_sparse, the
+// integer rule nactv + nacte <= m/20 (api.h:150-153); pct > 0:
+// frac_threshold(pct)::is_sparse with its float and double roundings
+// (api.h:135-139).
+bool threshold_sparse(float pct, uint64_t nactv, uint64_t nacte, uint64_t m) {
+    if (pct <= 0.f) return nactv + nacte <= m / 20;
+    return (double)(float)(nactv + nacte) <= (double)(float)m / (100.0 / (double)pct);
+}
+// The largest nactv + nacte that threshold_sparse calls sparse.
+uint64_t threshold_limit(float pct, uint64_t m) {
+    if (pct <= 0.f) return m / 20;
+    return (uint64_t)((double)(float)m / (100.0 / (double)pct));
+}
+
 int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_frontier *out,
                      gpr_iter_stats *st) {
+    gpr_frontier *const out_arg = out;
     GPR_REQUIRE(desc->prop && desc->prop->dtype == GPR_U32 && desc->prop->g == g,
                 "edgemap: relax needs a u32 property of this graph");
     GPR_REQUIRE(desc->record == GPR_RECORD_NONE || out, "edgemap: record needs an output frontier");
     GPR_REQUIRE(!out || out != in, "edgemap: output frontier must differ from the input");
-    GPR_REQUIRE(desc->record != GPR_RECORD_METHOD_NE || (desc->prev && desc->prev->dtype == GPR_U32),
-                "edgemap: record method needs a u32 prev property");
+    GPR_REQUIRE(desc->record >= GPR_RECORD_NONE && desc->record <= GPR_RECORD_REDUCTION_OR_METHOD,
+                "edgemap: unknown record %d", desc->record);
+    const bool method = desc->record == GPR_RECORD_METHOD_NE || desc->record == GPR_RECORD_REDUCTION_OR_METHOD;
+    GPR_REQUIRE(!method || (desc->prev && desc->prev->dtype == GPR_U32 && desc->prev->g == g),
+                "edgemap: record method needs a u32 prev property of this graph");
     const uint32_t n = g->n;
     const uint64_t m = g->m;
     const int in_kind = in ? in->kind : GPR_FRONTIER_ALL;
@@ -474,14 +521,12 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     if (dir == GPR_DIR_POLICY) GPR_TRY(policy_decide(g, nactv, nacte, &dir, &okind));
     PolicyCtx pol = {};
     if (use_pol) pol = policy_ctx(g, nacte);
-    if (dir == GPR_DIR_AUTO) {
-        const float pct = desc->threshold_pct > 0.f ? desc->threshold_pct : 5.f;
-        // frac_threshold::is_sparse, api.h:135-139
-        const bool sparse = (float)((uint64_t)nactv + nacte) <= (float)m / (100.0f / pct);
-        dir = sparse ? GPR_DIR_PUSH : GPR_DIR_PULL;
-    }
+    if (dir == GPR_DIR_AUTO) dir = threshold_sparse(desc->threshold_pct, nactv, nacte, m) ? GPR_DIR_PUSH : GPR_DIR_PULL;
     if (okind == 0) okind = dir == GPR_DIR_PUSH ? (nacte > n ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST)
                                                 : GPR_FRONTIER_BITMAP;
+    // no record argument: nothing is activated and the output stays empty
+    // (csr_sparse_no_f / csc_sparse_aset_no_f, api.h:2249-2252, 2321-2322)
+    if (desc->record == GPR_RECORD_NONE) out = nullptr;
     if (!out) okind = 0;
     if (okind == GPR_FRONTIER_LIST) GPR_TRY(frontier_ensure_list(out));
     if (okind == GPR_FRONTIER_BITMAP) GPR_TRY(frontier_ensure_bits(out));
@@ -492,18 +537,21 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     // api::fusion: a sparse step that processes what it activates at once
     // (fusion_select(true), or default_fusion_select: >= 4096 edges,
     // api.h:227-248); the record and dst filter are implied by the relax
+    const uint64_t small = (uint64_t)tun().fusion_small_limit;
+    const bool fuse_small = desc->fusion == GPR_FUSION_SMALL && (uint64_t)nactv + nacte <= small;
     const bool fuse = dir == GPR_DIR_PUSH && in && desc->relax != GPR_RELAX_SETIF &&
-                      (desc->fusion == GPR_FUSION_ON || (desc->fusion == GPR_FUSION_DEFAULT && nacte >= 4096));
+                      (desc->fusion == GPR_FUSION_ON || (desc->fusion == GPR_FUSION_DEFAULT && nacte >= 4096) ||
+                       fuse_small);
     if (fuse) {
         GPR_HIP(hipEventRecord(g->ev0, g->stream));
         if (in_kind != GPR_FRONTIER_LIST) GPR_TRY(frontier_to_list(in));
         // with the direction left to the engine, a level that turns dense
-        // ends the fused step and is returned, so the next step can pull
+        // ends the fused step and is returned, so the next step can pull;
+        // small-level fusion also hands back the first large level
         uint64_t dense_at = 0;
-        if (desc->direction == GPR_DIR_AUTO || desc->direction == GPR_DIR_POLICY) {
-            const float pct = desc->threshold_pct > 0.f ? desc->threshold_pct : 5.f;
-            dense_at = (uint64_t)((double)m / (100.0 / pct));
-        }
+        if (desc->direction == GPR_DIR_AUTO || desc->direction == GPR_DIR_POLICY)
+            dense_at = threshold_limit(desc->threshold_pct, m);
+        if (fuse_small) dense_at = dense_at ? std::min(dense_at, small) : small;
         GPR_TRY(fused_push(g, desc->relax, (uint32_t *)desc->prop->d, in, out, dense_at, &s));
         GPR_HIP(hipEventRecord(g->ev1, g->stream));
         GPR_HIP(hipEventSynchronize(g->ev1));
@@ -534,7 +582,11 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, in->list, k, g->deg, g->scan,
                            g->scan_blk);
         hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
-        uint64_t *bits = g->dedup;
+        // activations are deduplicated on the output bitmap, or for a list
+        // output on the graph's dedup bitmap, which is cleared again from
+        // the list afterwards; without an output nothing is activated, so
+        // the dedup bitmap stays clear
+        uint64_t *bits = okind == GPR_FRONTIER_LIST ? g->dedup : nullptr;
         if (okind == GPR_FRONTIER_BITMAP) {
             GPR_TRY(fill_words(g, out->bits, nwords(n), 0ull));
             bits = out->bits;
@@ -544,7 +596,8 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
             const unsigned grid = grid_for(E, PUSH_TILE, (unsigned)std::max(1.0, tun().push_blocks_per_cu * num_cus()));
 #define PUSH(R)                                                                                                \
     hipLaunchKernelGGL(k_push<R>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k, g->scan, g->scan_blk, E, \
-                       g->index, g->nbrs, g->deg, prop, desc->dst_filter, desc->dst_arg, bits, olist, g->ctr, pol)
+                       g->index, g->nbrs, g->deg, prop, desc->dst_filter, desc->dst_arg, desc->record, prev, bits, \
+                       olist, g->ctr, pol)
             if (desc->relax == GPR_RELAX_MIN_PLUS1) PUSH(GPR_RELAX_MIN_PLUS1);
             else if (desc->relax == GPR_RELAX_MIN) PUSH(GPR_RELAX_MIN);
             else PUSH(GPR_RELAX_SETIF);
@@ -571,6 +624,11 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     GPR_TRY(read_counters(g));
     float ms = 0.f;
     GPR_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+    if (desc->record == GPR_RECORD_NONE && out_arg) {
+        out_arg->kind = GPR_FRONTIER_EMPTY;
+        out_arg->nactv = 0;
+        out_arg->nacte = 0;
+    }
     if (out) {
         out->kind = okind;
         out->nactv = okind == GPR_FRONTIER_LIST ? g->hctr->listpos : g->hctr->nactv;
@@ -644,6 +702,18 @@ extern "C" int gpr_vertex_map(gpr_graph *g, int op, gpr_prop *p, const gpr_prop 
         else if (f->kind == GPR_FRONTIER_BITMAP)
             hipLaunchKernelGGL(k_vmap_fill_bits32, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d, f->bits, n,
                                (uint32_t)value_bits);
+        break;
+    case GPR_VMAP_COPY_FRONTIER:
+        GPR_REQUIRE(p->elem == 4 && q && q->elem == 4 && q->g == g && f && f->g == g,
+                    "vertex_map copy over a frontier needs two u32 properties and a frontier");
+        if (f->kind == GPR_FRONTIER_ALL)
+            GPR_HIP(hipMemcpyAsync(p->d, q->d, 4 * (size_t)n, hipMemcpyDeviceToDevice, g->stream));
+        else if (f->kind == GPR_FRONTIER_LIST && f->nactv)
+            hipLaunchKernelGGL(k_vmap_copy_list32, dim3(grid_for(f->nactv, 256, 8192)), dim3(256), 0, g->stream,
+                               (uint32_t *)p->d, (const uint32_t *)q->d, f->list, f->nactv);
+        else if (f->kind == GPR_FRONTIER_BITMAP)
+            hipLaunchKernelGGL(k_vmap_copy_bits32, dim3(grid), dim3(256), 0, g->stream, (uint32_t *)p->d,
+                               (const uint32_t *)q->d, f->bits, n);
         break;
     default:
         set_error("vertex_map: unknown op %d", op);

@@ -345,8 +345,13 @@ extern "C" int gpr_graph_create(const uint64_t *index, const uint32_t *nbrs, uin
                 for (uint64_t e = index[s]; e < index[s + 1]; ++e) cnb[fill[nbrs[e]]++] = s;
             if ((r = dev_alloc((void **)&g->cindex, sizeof(uint64_t) * ((size_t)n + 1), "CSC index"))) break;
             if ((r = dev_alloc((void **)&g->cnbrs, sizeof(uint32_t) * std::max<uint64_t>(m, 1), "CSC neighbours"))) break;
-            hipMemcpy(g->cindex, cidx.data(), sizeof(uint64_t) * ((size_t)n + 1), hipMemcpyHostToDevice);
-            if (m) hipMemcpy(g->cnbrs, cnb.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice);
+            if (hipMemcpy(g->cindex, cidx.data(), sizeof(uint64_t) * ((size_t)n + 1), hipMemcpyHostToDevice) !=
+                    hipSuccess ||
+                (m && hipMemcpy(g->cnbrs, cnb.data(), sizeof(uint32_t) * m, hipMemcpyHostToDevice) != hipSuccess)) {
+                set_error("upload of the CSC failed");
+                r = GPR_E_HIP;
+                break;
+            }
             hc = cidx.data();
         }
         r = finish_graph(g, index, hc);

@@ -68,6 +68,7 @@ struct Tunables {
     double fusion_spin_limit = 1 << 22;       // fused push: polls without progress before it gives up
     double fusion_blocks = 0;                 // fused push: cooperative grid size (0: one per CU)
     double pr_write_stage_cells = 4096;       // PR write: a tile's run offsets go to LDS up to this many cells
+    double fusion_small_limit = 1 << 17;      // GPR_FUSION_SMALL: frontier size that fuses / level size that ends it
 };
 Tunables &tun();
 int num_cus();
@@ -287,5 +288,9 @@ int map_ids(gpr_graph *g, uint32_t *d_ids, uint32_t k, bool to_internal);
 int to_internal_id(gpr_graph *g, uint32_t orig, uint32_t *internal);
 int deterministic_sum(gpr_graph *g, const double *parts, uint32_t count,
                       double *dst);               // device-side, async
+// push/pull switch (edgemap.hip): pct <= 0 is default_threshold (m/20),
+// pct > 0 frac_threshold(pct)
+bool threshold_sparse(float pct, uint64_t nactv, uint64_t nacte, uint64_t m);
+uint64_t threshold_limit(float pct, uint64_t m);
 
 } // namespace gpr

@@ -237,7 +237,7 @@ extern "C" int gpr_msbfs(gpr_graph *g, const uint32_t *sources, int k, const gpr
     const uint32_t n = g->n;
     for (int j = 0; j < k; ++j) GPR_REQUIRE(sources[j] < n, "msbfs: source %u out of range", sources[j]);
     const int dir_opt = opts ? opts->direction : GPR_DIR_AUTO;
-    const float pct = opts && opts->threshold_pct > 0 ? opts->threshold_pct : 5.f;
+    const float pct = opts ? opts->threshold_pct : 0.f;   // 0: BFSLVLv's default_threshold
     uint32_t *lev = nullptr, *src = nullptr, *list[2] = {nullptr, nullptr};
     uint64_t *seen = nullptr, *cur = nullptr, *nxt = nullptr;
     int r = GPR_OK;
@@ -286,8 +286,7 @@ extern "C" int gpr_msbfs(gpr_graph *g, const uint32_t *sources, int k, const gpr
         uint32_t level = 0;
         while (nactv > 0) {
             const bool sparse = dir_opt == GPR_DIR_PUSH ||
-                                (dir_opt != GPR_DIR_PULL &&
-                                 (float)((uint64_t)nactv + nacte) <= (float)g->m / (100.0f / pct));
+                                (dir_opt != GPR_DIR_PULL && threshold_sparse(pct, nactv, nacte, g->m));
             if (sparse) {
                 if ((r = scan_frontier_degrees(g, list[cl], nactv))) break;
                 if (nacte)

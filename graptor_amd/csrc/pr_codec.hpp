@@ -24,6 +24,8 @@ template <> struct Codec<GPR_CONTRIB_BF16> {
     __device__ static float dec(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
     __device__ static uint16_t enc(float f) {
         uint32_t u = __float_as_uint(f);
+        // a NaN stays a (quiet) NaN instead of rounding into the infinity
+        if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
         u += 0x7fffu + ((u >> 16) & 1u);
         return (uint16_t)(u >> 16);
     }

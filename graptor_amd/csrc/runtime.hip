@@ -204,6 +204,7 @@ static double *tunable(const char *name) {
     if (!strcmp(name, "fusion_spin_limit")) return &t.fusion_spin_limit;
     if (!strcmp(name, "fusion_blocks")) return &t.fusion_blocks;
     if (!strcmp(name, "pr_write_stage_cells")) return &t.pr_write_stage_cells;
+    if (!strcmp(name, "fusion_small_limit")) return &t.fusion_small_limit;
     return nullptr;
 }
 

@@ -24,9 +24,14 @@
 //   -ef E, -seed X     RMAT edge factor (16) and seed (0x47524150)
 //   -reorder           relabel by decreasing degree (GraphVEBOGraptor's role)
 //   -dir D             auto | push | pull | policy  (auto = m/20 threshold)
-//   -threshold T       frac_threshold(T) percent (5)
+//   -threshold T       frac_threshold(T) percent (default: the benchmark's own
+//                      rule, default_threshold m/20 for BFS, frac_threshold(5) for CC)
 //   -fusion F          fused sparse steps: default|off|1|2 (the benchmarks'
-//                      FUSION setting, BFSLVLv.C:28-40, CCv.C:47-54)
+//                      FUSION setting, BFSLVLv.C:28-40, CCv.C:47-54) or small
+//                      (small levels only, no reference counterpart)
+//   -deferred 0|1      the benchmarks' DEFERRED_UPDATE record forms (0: the
+//                      reduction; 1: method / reduction_or_method with a
+//                      maintained previous copy, BFSLVLv.C:10-12, CCv.C:26-28)
 //   -policy F          network weights (JSON with a "params" array) for -dir policy
 //   -pr:stages L       PR_vary / APR_vary stages fmt:tol,... with fmt
 //                      f32|bf16|f16|cfp16|bf16t|scfp16 (default cfp16:1e-3,f32:1e-7
@@ -111,7 +116,7 @@ void usage() {
                  "                  [-pr:maxiter K] [-pr:outfile F] [-cc:outfile F] [-kc:outfile F] [-itimes]\n"
                  "                  [-apr:eps2 E] [-kc:buckets B]\n"
                  "                  [-rmat S [-ef E] [-seed X]] [-reorder] [-dir auto|push|pull|policy]\n"
-                 "                  [-threshold T] [-fusion default|off|1|2] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
+                 "                  [-threshold T] [-fusion default|off|1|2|small] [-deferred 0|1] [-policy F] [-pr:stages fmt:tol,...] [-device N] [-tuned F]\n"
                  "                  <inFile>\n";
 }
 
@@ -425,13 +430,14 @@ int run(const Args &a) {
                 : dir == "policy" ? GPR_DIR_POLICY
                 : -1;
     if (o.direction < 0) throw std::runtime_error("unknown -dir " + dir);
-    o.threshold_pct = (float)a.real("-threshold", 5.0);
+    o.threshold_pct = (float)a.real("-threshold", 0.0);   // 0: the benchmark's own rule
     o.max_iter = (int)a.num("-pr:maxiter", 100);
     o.tol = 1e-7;
     o.trace = a.has("-itimes") ? 1 : 0;
     const std::string fu = a.str("-fusion", "default");
-    o.fusion = fu == "default" ? 0 : fu == "off" ? -1 : fu == "1" ? 1 : fu == "2" ? 2 : -2;
+    o.fusion = fu == "default" ? 0 : fu == "off" ? -1 : fu == "1" ? 1 : fu == "2" ? 2 : fu == "small" ? 3 : -2;
     if (o.fusion == -2) throw std::runtime_error("unknown -fusion " + fu);
+    o.deferred_update = (int)a.num("-deferred", 0) != 0;
     if (a.has("-pr:infile")) throw std::runtime_error("-pr:infile is not supported: runs start from x = 1/n");
 
     CALL(gpr_init((int)a.num("-device", 0)));

@@ -52,8 +52,9 @@ int gpr_init(int device);
 
 /* ---- graphs: GraphCSx (graph/GraphCSx.h:293-2806) ------------------------ */
 /* Upload a CSR (index has n+1 entries, nbrs m).  For non-symmetric graphs
- * the CSC is built on the device.  Replaces GraphCSx(index, edges) and
- * build_degree (GraphCSx.h:2797-2805). */
+ * the CSC is built on the host (a counting sort over the caller's CSR,
+ * rows ascending) and uploaded beside it.  Replaces GraphCSx(index, edges)
+ * and build_degree (GraphCSx.h:2797-2805). */
 int gpr_graph_create(const uint64_t *index, const uint32_t *nbrs, uint32_t n,
                      uint64_t m, int symmetric, gpr_graph **out);
 /* Read a b2 file (versions 2 and 3) -- GraphCSx::readFromBinaryFile,
@@ -142,9 +143,20 @@ enum {                      /* destination filter, api::filter(dst) */
     GPR_DST_NE = 2           /* p[d] != arg   CCv.C:559-564 */
 };
 enum {                      /* api::record (api.h:1450-1516) */
-    GPR_RECORD_NONE = 0,
-    GPR_RECORD_REDUCTION = 1, /* activate d when relax changed p[d] */
-    GPR_RECORD_METHOD_NE = 2  /* activate d when p[d] != prev[d] */
+    GPR_RECORD_NONE = 0,      /* no record argument: no output frontier
+                                 (csr_sparse_no_f, api.h:2249-2252) */
+    GPR_RECORD_REDUCTION = 1, /* api::reduction: activate d when a relax
+                                 changed p[d] */
+    GPR_RECORD_METHOD_NE = 2, /* api::method with [&](d){ return p[d] !=
+                                 prev[d]; }: pull activates every d with
+                                 p[d] != prev[d] after its row; push
+                                 activates d when a relax changed p[d] and
+                                 p[d] != prev[d] right after that relax
+                                 (csr_sparse_with_f: vexpr && uexpr,
+                                 dsl/edgemap.h:1497-1507) */
+    GPR_RECORD_REDUCTION_OR_METHOD = 3 /* api::reduction_or_method
+                                 (api.h:1401-1416): the method in push, the
+                                 reduction in pull (CCv.C:536-547) */
 };
 enum {                      /* traversal direction / threshold */
     GPR_DIR_AUTO = 0,        /* threshold rule, api.h:130-160 */
@@ -165,14 +177,18 @@ enum {                      /* contribution storage for GPR_RELAX_PR_SUM */
 typedef struct {
     int relax;            /* GPR_RELAX_* */
     gpr_prop *prop;       /* relaxed property (u32) */
-    gpr_prop *prev;       /* previous copy for GPR_RECORD_METHOD_NE */
+    gpr_prop *prev;       /* previous copy for GPR_RECORD_METHOD_NE and
+                             GPR_RECORD_REDUCTION_OR_METHOD */
     int src_strong;       /* api::filter(strong|weak, src, F) */
     int dst_filter;       /* GPR_DST_* */
     uint32_t dst_arg;
     int record;           /* GPR_RECORD_* */
     int direction;        /* GPR_DIR_* */
     int out_kind;         /* 0 = auto, GPR_FRONTIER_LIST / _BITMAP */
-    float threshold_pct;  /* frac_threshold(t); 5 = default m/20 */
+    float threshold_pct;  /* > 0: frac_threshold(t), float(nactv + nacte) <=
+                             float(m) / (100.0 / t) (api.h:135-139); 0:
+                             default_threshold, the integer rule
+                             nactv + nacte <= m / 20 (api.h:150-153) */
     /* GPR_RELAX_PR_SUM only: prop is the f32 accumulator y, contrib the
      * stored contributions in format contrib_fmt (GPR_CONTRIB_*). */
     gpr_prop *contrib;
@@ -187,7 +203,13 @@ typedef struct {
     int fusion;
 } gpr_emap_desc;
 
-enum { GPR_FUSION_OFF = 0, GPR_FUSION_ON = 1, GPR_FUSION_DEFAULT = 2 };
+/* GPR_FUSION_SMALL (no reference counterpart): a sparse step fuses when its
+ * frontier's vertices + out-edges are at most the tunable
+ * fusion_small_limit, and its levels run until one exceeds that limit,
+ * which it hands back as its output frontier (a list): small levels, whose
+ * unfused steps cost more in launches and round trips than in work, run in
+ * one launch, large ones as unfused steps. */
+enum { GPR_FUSION_OFF = 0, GPR_FUSION_ON = 1, GPR_FUSION_DEFAULT = 2, GPR_FUSION_SMALL = 3 };
 
 typedef struct {
     uint32_t nactv_in;
@@ -203,7 +225,9 @@ typedef struct {
 } gpr_iter_stats;
 
 /* One edgemap step over frontier `in`, recording into `out` (which may be
- * NULL when record is GPR_RECORD_NONE).  `in` may be NULL for all-true. */
+ * NULL when record is GPR_RECORD_NONE).  `in` may be NULL for all-true.
+ * With GPR_RECORD_NONE no vertex is activated and `out`, if given, is left
+ * empty. */
 int gpr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in,
                 gpr_frontier *out, gpr_iter_stats *st);
 
@@ -213,7 +237,9 @@ enum {
     GPR_VMAP_FILL = 0,       /* p[v] = value */
     GPR_VMAP_COPY = 1,       /* p[v] = q[v]   (maintain_copies :2515-2535) */
     GPR_VMAP_IOTA = 2,       /* p[v] = v */
-    GPR_VMAP_FILL_FRONTIER = 3 /* p[v] = value for v in F */
+    GPR_VMAP_FILL_FRONTIER = 3, /* p[v] = value for v in F */
+    GPR_VMAP_COPY_FRONTIER = 4  /* p[v] = q[v] for v in F (maintain_copies
+                                   over a sparse change set, :2515-2535) */
 };
 int gpr_vertex_map(gpr_graph *g, int op, gpr_prop *p, const gpr_prop *q,
                    gpr_frontier *f, uint64_t value_bits);
@@ -241,7 +267,9 @@ int gpr_pr_scalars(gpr_graph *g, double *sum_y, double *delta, double *sum_x,
 typedef struct {
     int direction;        /* GPR_DIR_* (AUTO = Graptor threshold) */
     int out_kind;         /* 0 = auto */
-    float threshold_pct;  /* 5 = m/20 */
+    float threshold_pct;  /* 0: the benchmark's own rule (BFSLVLv, BFSv and
+                             the 64-source BFS: default_threshold; CCv:
+                             frac_threshold(5)); > 0: frac_threshold(t) */
     int max_iter;         /* PR: -pr:maxiter (default 100) */
     double tol;           /* PR: 1e-7 */
     int trace;            /* record per-iteration stats (-itimes) */
@@ -249,8 +277,17 @@ typedef struct {
      * CCv.C:47-54): 0 = the reference's default for the benchmark (BFSLVLv
      * off, CCv FUSION 1), -1 = off, 1 = FUSION 1 (BFSLVLv: on for
      * low-degree graphs and after a dense step; CCv: default_fusion_select),
-     * 2 = FUSION 2 (every sparse step) */
+     * 2 = FUSION 2 (every sparse step), 3 = small levels only
+     * (GPR_FUSION_SMALL) */
     int fusion;
+    /* the benchmarks' DEFERRED_UPDATE setting (BFSLVLv.C:10-12, CCv.C:26-28):
+     * 0 = api::record(output, api::reduction) (the engine's default);
+     * 1 = the reference's default: BFSLVLv records the method
+     * level[d] != prev_level[d] and keeps prev_level with maintain_copies
+     * (BFSLVLv.C:166-172, 218-220); CCv records reduction_or_method with
+     * IDs[d] != prevIDs[d] and copies prevIDs = IDs after every step
+     * (CCv.C:536-549, 578-580).  Levels and labels are the same either way. */
+    int deferred_update;
 } gpr_run_opts;
 
 typedef struct {
@@ -385,6 +422,9 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
  *                       the step fails with GPR_E_HIP (default 2^22)
  *   fusion_blocks       fused push: blocks of 1024 threads in its
  *                       cooperative grid (0: one per compute unit)
+ *   fusion_small_limit  GPR_FUSION_SMALL: the largest frontier (vertices +
+ *                       out-edges) that fuses, and the level size that ends
+ *                       a fused step (default 2^17)
  *   pr_write_stage_cells  PageRank binned edgemap: a source tile's run
  *                       offsets are staged in LDS when it has at most this
  *                       many non-empty (tile, bin) cells (capped at 4096,

@@ -190,13 +190,32 @@ inline filter_dst_arg filter(strength s, dst_t, dst_pred p) { return {s, p}; }
 
 struct reduction_t {};
 constexpr reduction_t reduction{};
+struct reduction_or_method_t {};
+constexpr reduction_or_method_t reduction_or_method{};
+// The record method [&](auto d){ return p[d] != prev[d]; } (BFSLVLv.C:169-172,
+// CCv.C:538-540), the one method form the engine evaluates on the device.
+struct method_ne {
+    gpr_prop *p;
+    gpr_prop *prev;
+};
+inline method_ne differs(vertexprop<uint32_t> &p, vertexprop<uint32_t> &prev) { return {p.get(), prev.get()}; }
 struct record_arg {
     frontier *out;
     int kind;
     strength s;
+    gpr_prop *prev;
 };
 inline record_arg record(frontier &out, reduction_t, strength s = strong) {
-    return {&out, GPR_RECORD_REDUCTION, s};
+    return {&out, GPR_RECORD_REDUCTION, s, nullptr};
+}
+// api::record(output, method, strength) (api.h:1480-1496)
+inline record_arg record(frontier &out, method_ne m, strength s = strong) {
+    return {&out, GPR_RECORD_METHOD_NE, s, m.prev};
+}
+// api::record(output, api::reduction_or_method, method, strength)
+// (api.h:1509-1516): the method in push, the reduction in pull
+inline record_arg record(frontier &out, reduction_or_method_t, method_ne m, strength s = strong) {
+    return {&out, GPR_RECORD_REDUCTION_OR_METHOD, s, m.prev};
 }
 
 struct relax_arg {
@@ -229,9 +248,10 @@ struct learned_policy_t {};
 constexpr always_sparse_t always_sparse{};
 constexpr always_dense_t always_dense{};
 constexpr learned_policy_t learned_policy{};
+// pct 0: default_threshold (api.h:150-153); > 0: frac_threshold(pct)
 struct config_arg {
     int dir = GPR_DIR_AUTO;
-    float pct = 5.f;
+    float pct = 0.f;
     int out_kind = 0;
 };
 inline config_arg config(frac_threshold t) {
@@ -309,6 +329,11 @@ public:
         ops_.push_back({GPR_VMAP_COPY, dst.get(), src.get(), nullptr, 0});
         return *this;
     }
+    // vertex_map(F, [&](auto v){ return dst[v] = src[v]; })
+    lazy_executor &vertex_map_copy(frontier &F, vertexprop<uint32_t> &dst, const vertexprop<uint32_t> &src) {
+        ops_.push_back({GPR_VMAP_COPY_FRONTIER, dst.get(), src.get(), &F, 0});
+        return *this;
+    }
     // vertex_filter(G, F, out, p[v] == value) / != value
     lazy_executor &vertex_filter(frontier &F, frontier &out, vertexprop<uint32_t> &p, uint32_t value, bool neg) {
         filters_.push_back({&F, &out, p.get(), value, neg});
@@ -343,6 +368,19 @@ private:
 
 inline lazy_executor make_lazy_executor(const Graph &G) { return lazy_executor(G); }
 
+// maintain_copies (dsl/vertexmap.h:2515-2535): dst[v] = src[v] over the
+// change set, or over every vertex when it holds more than n/4 of them or
+// is not a vertex set.
+inline void maintain_copies(const Graph &G, frontier &change, vertexprop<uint32_t> &dst,
+                            const vertexprop<uint32_t> &src) {
+    const int kind = change.get() ? change.getType() : GPR_FRONTIER_ALL;
+    const bool do_all = kind == GPR_FRONTIER_ALL || change.nActiveVertices() > G.numVertices() / 4;
+    if (do_all)
+        make_lazy_executor(G).vertex_map_copy(dst, src).materialize();
+    else if (!change.isEmpty())
+        make_lazy_executor(G).vertex_map_copy(change, dst, src).materialize();
+}
+
 // ---- edgemap ------------------------------------------------------------------
 namespace detail {
 inline void apply(gpr_emap_desc &d, frontier *&in, frontier *&out, const filter_src_arg &a) {
@@ -355,6 +393,7 @@ inline void apply(gpr_emap_desc &d, frontier *&, frontier *&, const filter_dst_a
 }
 inline void apply(gpr_emap_desc &d, frontier *&, frontier *&out, const record_arg &a) {
     d.record = a.kind;
+    d.prev = a.prev;
     out = a.out;
 }
 inline void apply(gpr_emap_desc &d, frontier *&, frontier *&, const relax_arg &a) {
@@ -377,7 +416,7 @@ inline void apply(gpr_emap_desc &d, frontier *&, frontier *&, const fusion_arg &
 template <class... Args> lazy_executor edgemap(const Graph &G, Args &&...args) {
     gpr_emap_desc d = {};
     d.relax = -1;
-    d.threshold_pct = 5.f;
+    d.threshold_pct = 0.f;   // default_threshold unless api::config says otherwise
     frontier *in = nullptr, *out = nullptr;
     (detail::apply(d, in, out, args), ...);
     if (d.relax < 0) throw error(GPR_E_INVAL, "edgemap: must specify relax operation");

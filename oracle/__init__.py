@@ -243,11 +243,16 @@ def pr_fp64(index, nbrs, max_iter=20, tol=0.0, damping=0.85, outdeg=None):
     return x, it, d.value
 
 
+def default_threads():
+    """OMP_NUM_THREADS when set, else the CPUs this process may run on."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+
 def graptor_bfs(index, nbrs, start, threads=None):
     n = len(index) - 1
     out = np.empty(n, np.uint32)
     info = RunInfo()
-    lib().orc_graptor_bfs(n, len(nbrs), index, nbrs, start, out, threads or os.cpu_count(), C.byref(info))
+    lib().orc_graptor_bfs(n, len(nbrs), index, nbrs, start, out, threads or default_threads(), C.byref(info))
     return out, info
 
 
@@ -255,7 +260,7 @@ def graptor_cc(index, nbrs, threads=None):
     n = len(index) - 1
     out = np.empty(n, np.uint32)
     info = RunInfo()
-    lib().orc_graptor_cc(n, len(nbrs), index, nbrs, out, threads or os.cpu_count(), C.byref(info))
+    lib().orc_graptor_cc(n, len(nbrs), index, nbrs, out, threads or default_threads(), C.byref(info))
     return out, info
 
 
@@ -266,7 +271,7 @@ def graptor_pr(index, nbrs, max_iter=20, tol=0.0, threads=None, outdeg=None):
     x = np.empty(n, np.float32)
     d = C.c_double()
     it = lib().orc_graptor_pr(n, index, nbrs, np.ascontiguousarray(outdeg, np.uint32), max_iter, tol, x,
-                              threads or os.cpu_count(), C.byref(d))
+                              threads or default_threads(), C.byref(d))
     return x, it, d.value
 
 
@@ -321,7 +326,7 @@ def apr_vary(index, nbrs, stages=APR_STAGES_2046, max_iter=100, eps2=1e-2, threa
     tr = np.zeros((max_iter, 4), np.float64)
     it = lib().orc_apr_vary(n, len(cnbrs), np.ascontiguousarray(cindex, np.uint64),
                             np.ascontiguousarray(cnbrs, np.uint32), np.ascontiguousarray(outdeg, np.uint32),
-                            fm, ep, len(stages), max_iter, eps2, x, threads or os.cpu_count(), C.byref(l1),
+                            fm, ep, len(stages), max_iter, eps2, x, threads or default_threads(), C.byref(l1),
                             C.byref(ed), tr.ctypes.data)
     if it < 0:
         raise ValueError("apr_vary: contribution format without a sign")

@@ -168,7 +168,7 @@ def test_dense_level_handed_back_then_pulled():
         assert tr[0]["fused"] == 1 and tr[0]["nactv_out"] > 0 and tr[1]["direction"] == G.DIR_PULL, tr[:2]
         # the handed-back frontier is the first level above the threshold
         m = int(deg.sum())
-        lvl = next(L for L in range(1, n) if (ref == L).sum() + deg[ref == L].sum() > m / 20)
+        lvl = next(L for L in range(1, n) if (ref == L).sum() + deg[ref == L].sum() > m // 20)
         assert tr[0]["nactv_out"] == (ref == lvl).sum() and tr[0]["nacte_out"] == deg[ref == lvl].sum(), lvl
         lv, _ = g.bfslvl(s, fusion=FUSION_2, direction=G.DIR_PUSH, trace=True)
         assert np.array_equal(lv, ref)
