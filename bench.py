@@ -124,8 +124,8 @@ def run(args):
     none = G.RunInfo()
 
     def step(download=False):
-        lv, ib = g.bfslvl(src, download=download) if "bfs" in algos else (None, none)
-        lab, ic = g.ccv(download=download) if "cc" in algos else (None, none)
+        lv, ib = g.bfslvl(src, download=download, fusion=args.fusion) if "bfs" in algos else (None, none)
+        lab, ic = g.ccv(download=download, fusion=args.fusion) if "cc" in algos else (None, none)
         x, ip = g.prv(max_iter=args.pr_iters, tol=0.0, download=download) if "pr" in algos else (None, none)
         return (lv, ib), (lab, ic), (x, ip)
 
@@ -159,9 +159,9 @@ def run(args):
         return None
     # the BFS and CC runs once more with per-step traces, for their
     # algorithmic bytes
-    g.bfslvl(src, download=False, trace=True)
+    g.bfslvl(src, download=False, trace=True, fusion=args.fusion)
     b_bfs = run_bytes(g.trace(), n, "bfs")
-    g.ccv(download=False, trace=True)
+    g.ccv(download=False, trace=True, fusion=args.fusion)
     b_cc = run_bytes(g.trace(), n, "cc")
     total_edges = eb + ec + ep
     value = total_edges / wall / 1e9
@@ -375,6 +375,8 @@ def main():
     ap.add_argument("--pr-iters", type=int, default=20)
     ap.add_argument("--cpu-pr-iters", type=int, default=5)
     ap.add_argument("--cpu-rounds", type=int, default=5, help="CPU baseline: median of this many runs")
+    ap.add_argument("--fusion", type=int, default=0,
+                    help="BFSLVLv/CCv fused steps: 0 = each benchmark's reference default, -1 = off, 1, 2, 3 = small")
     ap.add_argument("--algos", default="bfs,cc,pr",
                     help="run a subset of the step (for counter runs; prints timings only)")
     ap.add_argument("--no-cpu", action="store_true")
