@@ -121,6 +121,7 @@ SIGNATURES = {
     "gpr_kcore": (C.c_int, [_P, C.POINTER(RunOpts), _P, C.POINTER(C.c_uint32), C.POINTER(RunInfo)]),
     "gpr_apr_vary": (C.c_int, [_P, C.POINTER(PrStage), C.c_int, C.c_double, C.POINTER(RunOpts), _P,
                                C.POINTER(RunInfo)]),
+    "gpr_batch_cc_noalloc": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     "gpr_msbfs": (C.c_int, [_P, _P, C.c_int, C.POINTER(RunOpts), _P, C.POINTER(RunInfo)]),
     "gpr_batch_generate": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _PP]),
     "gpr_batch_create": (C.c_int, [_P, _P, _P, C.c_uint32, _PP]),

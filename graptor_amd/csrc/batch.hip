@@ -21,6 +21,7 @@
 #include "device.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -316,9 +317,65 @@ static void batch_sizes(uint32_t count, uint32_t nmin, uint32_t nmax, uint64_t s
     }
 }
 
+// ---- the entry point that allocates nothing ----------------------------------
+// gpr_batch_cc_noalloc's scratch, one copy per GPU for the life of the
+// process: word 0 is the work-queue counter (cleared by k_batch_reset at
+// every call); from word NOALLOC_PARENTS on, the union-find parents of the
+// graphs above the LDS capacity, indexed by global vertex ID, so a batch may
+// hold at most NOALLOC_MAX_V vertices.  Host code reaches the array only
+// through the address hipGetSymbolAddress returns for the current device.
+constexpr uint32_t NOALLOC_MAX_V = 1u << 21;
+constexpr uint32_t NOALLOC_PARENTS = 64;
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static __device__ __align__(256) uint32_t scratch[NOALLOC_PARENTS + NOALLOC_MAX_V];
+
+struct NoallocDevice {
+    std::atomic<uint32_t *> ptr{nullptr};
+    std::atomic<int> cus{0};
+};
+static NoallocDevice noalloc_dev[NOALLOC_MAX_DEVICES];
+
+// Once per device, with that device current: the device address of
+// `scratch` and the compute-unit count (gpr_init, else the first call).
+int batch_noalloc_setup(int dev) {
+    GPR_REQUIRE(dev >= 0 && dev < NOALLOC_MAX_DEVICES, "device %d: no scratch slot", dev);
+    NoallocDevice &d = noalloc_dev[dev];
+    if (d.ptr.load(std::memory_order_acquire)) return GPR_OK;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        return GPR_E_HIP;
+    }
+    int cus = 0;
+    GPR_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    d.cus.store(cus > 0 ? cus : 1, std::memory_order_relaxed);
+    d.ptr.store((uint32_t *)p, std::memory_order_release);
+    return GPR_OK;
+}
+
 } // namespace gpr
 
 using namespace gpr;
+
+extern "C" int gpr_batch_cc_noalloc(const uint64_t *d_index, const uint32_t *d_nbrs, const uint32_t *d_voff,
+                                    uint32_t count, uint32_t n, uint32_t *d_labels, void *stream) {
+    GPR_REQUIRE(d_index && d_voff && d_labels && (d_nbrs || n == 0) && count > 0,
+                "batch_cc_noalloc: null buffer or empty batch");
+    GPR_REQUIRE(n <= NOALLOC_MAX_V, "batch_cc_noalloc: %u vertices exceed the limit of %u", n, NOALLOC_MAX_V);
+    int dev = 0;
+    GPR_HIP(hipGetDevice(&dev));
+    GPR_TRY(batch_noalloc_setup(dev));
+    uint32_t *sp = noalloc_dev[dev].ptr.load(std::memory_order_acquire);
+    const int cus = noalloc_dev[dev].cus.load(std::memory_order_relaxed);
+    hipStream_t st = (hipStream_t)stream;
+    const unsigned grid = (unsigned)std::max(1, std::min<int>(cus, (int)std::min<uint32_t>(count, 1u << 30)));
+    hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, (unsigned int *)sp);
+    hipLaunchKernelGGL(k_batch_cc, dim3(grid), dim3(BCC_THREADS), 0, st, count, n, (const uint32_t *)nullptr, d_voff,
+                       d_index, d_nbrs, d_labels, sp + NOALLOC_PARENTS, (unsigned int *)sp);
+    GPR_HIP(hipGetLastError());
+    return GPR_OK;
+}
 
 extern "C" int gpr_batch_generate(uint32_t count, uint32_t nmin, uint32_t nmax, uint64_t seed, gpr_batch **out) {
     if (!out || count == 0 || nmin < 2 || nmax < nmin || nmax > (1u << 24)) {

@@ -288,6 +288,7 @@ int map_ids(gpr_graph *g, uint32_t *d_ids, uint32_t k, bool to_internal);
 int to_internal_id(gpr_graph *g, uint32_t orig, uint32_t *internal);
 int deterministic_sum(gpr_graph *g, const double *parts, uint32_t count,
                       double *dst);               // device-side, async
+int batch_noalloc_setup(int dev);                 // scratch address of gpr_batch_cc_noalloc (batch.hip)
 // push/pull switch (edgemap.hip): pct <= 0 is default_threshold (m/20),
 // pct > 0 frac_threshold(pct)
 bool threshold_sparse(float pct, uint64_t nactv, uint64_t nacte, uint64_t m);

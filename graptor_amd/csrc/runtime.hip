@@ -192,7 +192,7 @@ extern "C" int gpr_init(int device) {
         return GPR_E_NODEV;
     }
     GPR_HIP(hipSetDevice(device));
-    return GPR_OK;
+    return batch_noalloc_setup(device);
 }
 
 static double *tunable(const char *name) {

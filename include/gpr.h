@@ -376,6 +376,22 @@ int gpr_batch_download(const gpr_batch *b, uint64_t *index, uint32_t *nbrs,
  * hipGraph launch (queue reset + persistent kernel). */
 int gpr_batch_cc(gpr_batch *b, uint32_t *labels, gpr_run_info *info);
 int gpr_batch_destroy(gpr_batch *b);
+/* gpr_batch_cc on a batch in caller-owned device memory, allocating nothing
+ * (no hipMalloc, hipFree or allocating library call).  No counterpart in the
+ * reference: it is for callers that cannot allocate, such as code under HIP
+ * graph capture or a serving loop.  d_index (n+1 u64), d_nbrs (m u32),
+ * d_voff (count+1 u32, d_voff[count] = n) and d_labels (n u32) are device
+ * pointers on the current device, laid out as for gpr_batch_create (every
+ * edge inside its graph); stream is a hipStream_t (NULL: the null stream).
+ * The kernels are enqueued and the call returns without synchronising;
+ * d_labels then receives what gpr_batch_cc returns.  Graphs are taken in
+ * index order.  The work-queue counter and the parents of graphs above
+ * 65536 vertices live in a static device array of each GPU, so a batch may
+ * hold at most 2^21 vertices (GPR_E_INVAL above that).  Calls that overlap
+ * on one GPU share that array and race. */
+int gpr_batch_cc_noalloc(const uint64_t *d_index, const uint32_t *d_nbrs,
+                         const uint32_t *d_voff, uint32_t count, uint32_t n,
+                         uint32_t *d_labels, void *stream);
 
 /* ---- learned direction / representation policy (GPR_DIR_POLICY) --------
  * No counterpart in the reference, whose rule is the fixed threshold of
