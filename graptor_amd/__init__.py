@@ -47,7 +47,7 @@ class EmapDesc(C.Structure):
 class IterStats(C.Structure):
     _fields_ = [("nactv_in", C.c_uint32), ("nacte_in", C.c_uint64), ("nactv_out", C.c_uint32),
                 ("nacte_out", C.c_uint64), ("direction", C.c_int), ("out_kind", C.c_int),
-                ("edges_scanned", C.c_uint64), ("ms", C.c_double), ("fused", C.c_int)]
+                ("edges_scanned", C.c_uint64), ("ms", C.c_double), ("fused", C.c_int), ("levels", C.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -380,8 +380,10 @@ class Graph:
         p = np.ascontiguousarray(params, np.float32)
         _check(lib().gpr_policy_set(self.h, _ptr(p), len(p)))
 
-    def policy_trace(self, algo, start=0, cap=4096):
-        """Per-iteration features (k,6) and timed choices (k,4)."""
+    def policy_trace(self, algo, start=0, cap=64):
+        """Features (k,6) of up to `cap` evenly spread states of a BFS
+        (algo 0, from `start`) or CC (algo 1) run, and the ms to finish the
+        run from each state under each of the four actions (k,4)."""
         f = np.empty((cap, 6), np.float32)
         t = np.empty((cap, 4), np.float32)
         k = C.c_int()
@@ -536,6 +538,13 @@ class Frontier:
 
     def convert(self, kind):
         _check(lib().gpr_frontier_convert(self.h, kind))
+
+    def filter(self, prop, value, neg=False, out=None):
+        """vertex_filter: the vertices of this frontier with prop[v] == value
+        (!= with neg), into `out` (a new frontier when None)."""
+        out = out or Frontier(self.g)
+        _check(lib().gpr_vertex_filter(self.g.h, self.h, prop.h, value, 1 if neg else 0, out.h))
+        return out
 
     def download(self):
         nactv = self.stats()[0]

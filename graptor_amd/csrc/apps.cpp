@@ -143,7 +143,8 @@ struct BFSLVLv {
             info->edges += output.nActiveEdges();
             F.del();
             F = output;
-            ++iter;
+            // a fused step hands back a frontier that many levels on
+            iter += lazy.stats().levels > 0 ? lazy.stats().levels : 1;
             // api.h:150-153 default_threshold::is_sparse on the new frontier
             if (fmode == 1 && !threshold_sparse(0.f, F.nActiveVertices(), F.nActiveEdges(), GA.numEdges()))
                 enable_fusion = true;

@@ -84,4 +84,34 @@ __device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *
     return lo;
 }
 
+// A wave's private queue of list entries in LDS (cap entries, cap a
+// multiple of 64 and >= 128).  Entries go in 64 lanes at a time, as a
+// ballot word; the queue is appended to the output list with ONE atomic on
+// the list position when the next word might not fit, and at the end, so
+// that a list of k entries takes about k / (cap - 64) atomics on that one
+// word instead of one per wave word.  The order of the entries in the
+// list is not specified.  All 64 lanes call push and flush together.
+struct WaveQueue {
+    uint32_t *q;     // this wave's cap entries of LDS
+    uint32_t cap;
+    uint32_t cnt;    // wave-uniform
+    __device__ __forceinline__ void flush(uint32_t *out, unsigned int *listpos) {
+        if (!cnt) return;
+        __builtin_amdgcn_wave_barrier();
+        uint32_t base = 0;
+        if (lane() == 0) base = atomicAdd(listpos, cnt);
+        base = __shfl(base, 0, 64);
+        for (uint32_t j = lane(); j < cnt; j += 64) out[base + j] = q[j];
+        __builtin_amdgcn_wave_barrier();
+        cnt = 0;
+    }
+    __device__ __forceinline__ void push(bool keep, uint64_t word, uint32_t v, uint32_t *out,
+                                         unsigned int *listpos) {
+        if (!word) return;
+        if (cnt + 64 > cap) flush(out, listpos);
+        if (keep) q[cnt + mbcnt(word)] = v;
+        cnt += (uint32_t)__popcll(word);
+    }
+};
+
 } // namespace gpr

@@ -27,6 +27,7 @@
 #include "internal.hpp"
 #include "device.hpp"
 #include "policy_dev.hpp"
+#include "grid_sync.hpp"
 
 #include <algorithm>
 
@@ -95,29 +96,47 @@ template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) 
 // is set (unset: ~0); the value is the source's (internal) ID
 constexpr uint32_t SETIF_OPEN = 0x80000000u;
 
-// Fused policy inference: the last workgroup to finish (by ticket) turns
-// the completed frontier counts into the next step's choice.
+// Fused policy inference: the first wave of the last workgroup to finish
+// turns the completed frontier counts into the next step's choice.  The
+// last one is found by tickets in two levels: a workgroup takes a ticket of
+// its counter stripe, and the last of a stripe one of the whole grid, so
+// no line takes more than one atomic per 64 workgroups (plus one per
+// stripe).  Each workgroup's counts are agent-scope atomics that its thread
+// 0 waits to be acknowledged (s_waitcnt vmcnt(0)) before its ticket, and
+// the last workgroup reads them with agent-scope loads: no fence, which on
+// this part would write back the XCD's L2 in every workgroup.  Each level's
+// last arrival resets its ticket for the next launch.
 __device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounters *ctr) {
     if (!pol.params) return;
     __shared__ bool last;
-    __shared__ PolicyLds lds;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(&ctr->ticket, 1u) == gridDim.x - 1;
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+        const unsigned sidx = blockIdx.x % CTR_STRIPES;
+        const unsigned nstripes = gridDim.x < (unsigned)CTR_STRIPES ? gridDim.x : (unsigned)CTR_STRIPES;
+        const unsigned ssize = gridDim.x / CTR_STRIPES + (sidx < gridDim.x % CTR_STRIPES ? 1u : 0u);
+        CtrStripe *st = ctr_stripe(ctr);
+        bool l = false;
+        if (atomicAdd(&st->ticket, 1u) == ssize - 1) {
+            __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            l = atomicAdd(&ctr->ticket, 1u) == nstripes - 1;
+            if (l) __hip_atomic_store(&ctr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        last = l;
     }
     __syncthreads();
-    if (!last) return;   // uniform across the workgroup
-    __threadfence();
-    const uint32_t nv = __hip_atomic_load(&ctr->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t ne = __hip_atomic_load(&ctr->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!last || threadIdx.x >= 64) return;   // the last workgroup's first wave
+    // the weights, then the striped counts (lane i reads stripe i): all
+    // loads in flight together
+    MlpWave net;
+    net.load(pol.params);
+    const CtrStripe *st = reinterpret_cast<const CtrStripe *>(ctr + 1) + lane();
+    const uint32_t nv = wave_sum(__hip_atomic_load(&st->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    const uint64_t ne = wave_sum(__hip_atomic_load(&st->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
     float f[6];
     policy_features(pol, nv, ne, f);
-    const int d = policy_forward_block(pol.params, f, lds);
-    if (threadIdx.x == 0) {
-        ctr->decision = (unsigned)d;
-        ctr->ticket = 0;
-    }
+    const int d = net.run(f, pol.margin);
+    if (threadIdx.x == 0) ctr->decision = (unsigned)d;
 }
 
 // ---- push ------------------------------------------------------------------
@@ -234,14 +253,16 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
     const uint32_t tv = block_sum(my_v, red32);
     const uint64_t te = block_sum(my_e, red);
     if (threadIdx.x == 0) {
-        if (tv) atomicAdd(&ctr->nactv, tv);
-        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+        if (tv) atomicAdd(&ctr_stripe(ctr)->nactv, tv);
+        if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
     }
     policy_epilogue(pol, ctr);
 }
 
 // ---- pull ------------------------------------------------------------------
 constexpr int PULL_THREADS = 256;
+constexpr uint32_t PULL_QCAP = 1024;   // list output: LDS queue entries per wave
+constexpr size_t PULL_QBYTES = sizeof(uint32_t) * PULL_QCAP * (PULL_THREADS / 64);
 constexpr uint32_t PULL_SERIAL_MAX = 32;
 // The destination filter usually stops a row after a few edges (early
 // exit), so the lane-serial path loads one neighbour at a time: loading
@@ -257,10 +278,12 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
        uint32_t *out_list, DevCounters *ctr, PolicyCtx pol, uint32_t serial_max) {
     __shared__ uint64_t red[PULL_THREADS / 64];
     __shared__ uint32_t red32[PULL_THREADS / 64];
+    extern __shared__ uint32_t pull_q[];   // list output: PULL_QCAP entries per wave
     const uint32_t ngroups = (n + 63) / 64;
     const uint32_t wid = (blockIdx.x * PULL_THREADS + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * (PULL_THREADS / 64);
     const unsigned ln = lane();
+    WaveQueue wq{pull_q + (threadIdx.x / 64) * PULL_QCAP, PULL_QCAP, 0};
     uint32_t my_v = 0;
     uint64_t my_e = 0, my_scan = 0;
     for (uint32_t grp = wid; grp < ngroups; grp += nwaves) {
@@ -347,22 +370,34 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
                                                           : false;
         const uint64_t word = __ballot(rec);
         if (out_bits && ln == 0) out_bits[grp] = word;
-        if (out_list && word) {
-            uint32_t base = 0;
-            if (ln == 0) base = atomicAdd(&ctr->listpos, (unsigned)__popcll(word));
-            base = __shfl(base, 0, 64);
-            if (rec) out_list[base + mbcnt(word)] = d;
-        }
+        if (out_list) wq.push(rec, word, d, out_list, &ctr->listpos);
         if (ln == 0) my_v += (uint32_t)__popcll(word);
         if (rec) my_e += deg[d];
+    }
+    if (out_list) {
+        // the waves' remaining queues go out together: one append per
+        // workgroup (out_list is the same for the whole grid)
+        __shared__ uint32_t qc[PULL_THREADS / 64], qbase;
+        const unsigned w = threadIdx.x / 64;
+        if (ln == 0) qc[w] = wq.cnt;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (int j = 0; j < PULL_THREADS / 64; ++j) t += qc[j];
+            qbase = t ? atomicAdd(&ctr->listpos, t) : 0u;
+        }
+        __syncthreads();
+        uint32_t off = qbase;
+        for (unsigned j = 0; j < w; ++j) off += qc[j];
+        for (uint32_t j = ln; j < wq.cnt; j += 64) out_list[off + j] = wq.q[j];
     }
     const uint32_t tv = block_sum(my_v, red32);
     const uint64_t te = block_sum(my_e, red);
     const uint64_t ts = block_sum(my_scan, red);
     if (threadIdx.x == 0) {
-        if (tv) atomicAdd(&ctr->nactv, tv);
-        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
-        if (ts) atomicAdd(&ctr->edges, (unsigned long long)ts);
+        if (tv) atomicAdd(&ctr_stripe(ctr)->nactv, tv);
+        if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
+        if (ts) atomicAdd(&ctr_stripe(ctr)->edges, (unsigned long long)ts);
     }
     policy_epilogue(pol, ctr);
 }
@@ -423,36 +458,65 @@ k_vfilter_bits(uint32_t n, const uint32_t *__restrict__ p, uint32_t value, int n
     const uint32_t tv = block_sum(my_v, red32);
     const uint64_t te = block_sum(my_e, red);
     if (threadIdx.x == 0) {
-        if (tv) atomicAdd(&ctr->nactv, tv);
-        if (te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+        if (tv) atomicAdd(&ctr_stripe(ctr)->nactv, tv);
+        if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
     }
 }
 
-__global__ void __launch_bounds__(256)
+// Vertex filter over a list, order-preserving, in two phases (as the
+// bitmap-to-list conversion): phase 0 counts the entries each workgroup
+// keeps among its VF_TILE consecutive list entries (wave w takes the w-th
+// run of 64 * VF_ITEMS) and adds their number and out-degree sum to the
+// counters; after an exclusive scan of those counts, phase 1 writes the
+// kept entries at their offsets.  No shared append position is touched.
+constexpr int VF_THREADS = 256, VF_ITEMS = 8, VF_TILE = VF_THREADS * VF_ITEMS;
+__global__ void __launch_bounds__(VF_THREADS)
 k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__restrict__ p, uint32_t value,
-               int neg, uint32_t *out, const uint32_t *__restrict__ deg, DevCounters *ctr) {
-    __shared__ uint64_t red[4];
-    uint64_t my_e = 0;
-    const uint32_t stride = gridDim.x * 256;
-    for (uint32_t b = blockIdx.x * 256; b < k; b += stride) {
-        const uint32_t i = b + threadIdx.x;
-        uint32_t v = 0;
-        bool keep = false;
-        if (i < k) {
-            v = in[i];
-            keep = (p[v] == value) != (neg != 0);
-        }
-        const uint64_t w = __ballot(keep);
-        if (w) {
-            uint32_t base = 0;
-            if (lane() == 0) base = atomicAdd(&ctr->listpos, (unsigned)__popcll(w));
-            base = __shfl(base, 0, 64);
-            if (keep) out[base + mbcnt(w)] = v;
-        }
-        if (keep) my_e += deg[v];
+               int neg, uint32_t *out, const uint32_t *__restrict__ deg, unsigned long long *blk, int phase,
+               DevCounters *ctr) {
+    __shared__ uint64_t red[VF_THREADS / 64];
+    __shared__ uint32_t wtot[VF_THREADS / 64];
+    const unsigned w = threadIdx.x / 64, ln = lane();
+    const uint64_t base = (uint64_t)blockIdx.x * VF_TILE + (uint64_t)w * 64 * VF_ITEMS;
+    uint32_t v[VF_ITEMS];
+#pragma unroll
+    for (int it = 0; it < VF_ITEMS; ++it) {
+        const uint64_t i = base + it * 64 + ln;
+        v[it] = i < k ? in[i] : 0u;
     }
-    const uint64_t te = block_sum(my_e, red);
-    if (threadIdx.x == 0 && te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+    uint64_t m[VF_ITEMS];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int it = 0; it < VF_ITEMS; ++it) {
+        const uint64_t i = base + it * 64 + ln;
+        const bool keep = i < k && (p[v[it]] == value) != (neg != 0);
+        m[it] = __ballot(keep);
+        cnt += (uint32_t)__popcll(m[it]);
+    }
+    if (ln == 0) wtot[w] = cnt;
+    if (phase == 0) {
+        uint64_t es = 0;
+#pragma unroll
+        for (int it = 0; it < VF_ITEMS; ++it)
+            if ((m[it] >> ln) & 1ull) es += deg[v[it]];
+        const uint64_t te = block_sum(es, red);   // its barriers also publish wtot
+        if (threadIdx.x == 0) {
+            uint32_t c = 0;
+            for (int j = 0; j < VF_THREADS / 64; ++j) c += wtot[j];
+            blk[blockIdx.x] = c;
+            if (c) atomicAdd(&ctr_stripe(ctr)->nactv, c);
+            if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
+        }
+        return;
+    }
+    __syncthreads();
+    uint64_t off = blk[blockIdx.x];
+    for (unsigned j = 0; j < w; ++j) off += wtot[j];
+#pragma unroll
+    for (int it = 0; it < VF_ITEMS; ++it) {
+        if ((m[it] >> ln) & 1ull) out[off + mbcnt(m[it])] = v[it];
+        off += (uint32_t)__popcll(m[it]);
+    }
 }
 
 // ---- host side ----------------------------------------------------------------
@@ -472,9 +536,11 @@ static unsigned pull_grid(uint32_t n) {
 }
 
 int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st);
-int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *out_kind);
+int policy_action(gpr_graph *g, uint32_t nactv, uint64_t nacte);
+void policy_log_features(gpr_graph *g, uint32_t nactv, uint64_t nacte);
+double host_clock_ms();
 PolicyCtx policy_ctx(gpr_graph *g, uint64_t nacte_in);
-void policy_after_step(gpr_graph *g, uint64_t nacte_in, uint32_t decision);
+void policy_after_step(gpr_graph *g, uint64_t nacte_in, int decision);
 
 // The push/pull switch.  pct <= 0: default_threshold::is_sparse, the
 // integer rule nactv + nacte <= m/20 (api.h:150-153); pct > 0:
@@ -510,6 +576,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     gpr_iter_stats s = {};
     s.nactv_in = nactv;
     s.nacte_in = nacte;
+    s.levels = 1;
     if (in_kind == GPR_FRONTIER_EMPTY || nactv == 0) {
         if (out) { out->kind = GPR_FRONTIER_EMPTY; out->nactv = 0; out->nacte = 0; }
         if (st) *st = s;
@@ -517,10 +584,54 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     }
     int dir = desc->direction;
     int okind = desc->out_kind;
-    const bool use_pol = dir == GPR_DIR_POLICY && g->pol_params;
-    if (dir == GPR_DIR_POLICY) GPR_TRY(policy_decide(g, nactv, nacte, &dir, &okind));
+    int fusion = desc->fusion;
+    // trace runs force one step's action (no inference)
+    const bool forced = dir == GPR_DIR_POLICY && g->pol_force_step >= 0;
+    const bool use_pol = dir == GPR_DIR_POLICY && (g->pol_params || forced);
+    if (dir == GPR_DIR_POLICY) {
+        // the learned policy's action; without a network, the rule
+        int action = 0;
+        if (forced) {
+            if (g->pol_log) policy_log_features(g, nactv, nacte);
+            if (g->pol_iter == g->pol_force_step) {
+                action = g->pol_force_action;
+                g->pol_force_t0 = host_clock_ms();
+            }
+        } else if (use_pol) {
+            action = policy_action(g, nactv, nacte);
+        }
+        switch (action) {
+        case 1:   // unfused sparse push; where the rule pulls, only while the
+                  // frontier's edges stay within twice its threshold (a push
+                  // of a larger frontier can cost many pulls)
+            if (threshold_sparse(desc->threshold_pct, nactv, nacte, m) ||
+                nactv + nacte <= 2 * threshold_limit(desc->threshold_pct, m)) {
+                dir = GPR_DIR_PUSH;
+                fusion = GPR_FUSION_OFF;
+            } else {
+                dir = GPR_DIR_AUTO;
+            }
+            break;
+        case 2:   // push fused while its levels stay small
+            dir = GPR_DIR_PUSH;
+            fusion = GPR_FUSION_SMALL;
+            break;
+        case 3:   // a pull the rule takes anyway, writing a list (a pull's
+                  // cost grows with n, which the features do not carry, so
+                  // the network may not turn a sparse step into one)
+            if (threshold_sparse(desc->threshold_pct, nactv, nacte, m)) {
+                dir = GPR_DIR_AUTO;
+            } else {
+                dir = GPR_DIR_PULL;
+                okind = GPR_FRONTIER_LIST;
+            }
+            break;
+        default:  // the threshold rule's own step
+            dir = GPR_DIR_AUTO;
+        }
+    }
     PolicyCtx pol = {};
-    if (use_pol) pol = policy_ctx(g, nacte);
+    if (use_pol && !forced) pol = policy_ctx(g, nacte);
     if (dir == GPR_DIR_AUTO) dir = threshold_sparse(desc->threshold_pct, nactv, nacte, m) ? GPR_DIR_PUSH : GPR_DIR_PULL;
     if (okind == 0) okind = dir == GPR_DIR_PUSH ? (nacte > n ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST)
                                                 : GPR_FRONTIER_BITMAP;
@@ -536,12 +647,16 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
                 "edgemap: push over the implicit all-true frontier needs an explicit frontier");
     // api::fusion: a sparse step that processes what it activates at once
     // (fusion_select(true), or default_fusion_select: >= 4096 edges,
-    // api.h:227-248); the record and dst filter are implied by the relax
+    // api.h:227-248); the record and dst filter are implied by the relax.
+    // The fused kernel runs one workgroup per CU, so a large level costs it
+    // more than an unfused step: under default_fusion_select a frontier
+    // above fusion_small_limit runs unfused, and a level above it ends the
+    // fused step, as under GPR_FUSION_SMALL
     const uint64_t small = (uint64_t)tun().fusion_small_limit;
-    const bool fuse_small = desc->fusion == GPR_FUSION_SMALL && (uint64_t)nactv + nacte <= small;
+    const bool small_in = (uint64_t)nactv + nacte <= small;
+    const bool fuse_small = (fusion == GPR_FUSION_SMALL || (fusion == GPR_FUSION_DEFAULT && nacte >= 4096)) && small_in;
     const bool fuse = dir == GPR_DIR_PUSH && in && desc->relax != GPR_RELAX_SETIF &&
-                      (desc->fusion == GPR_FUSION_ON || (desc->fusion == GPR_FUSION_DEFAULT && nacte >= 4096) ||
-                       fuse_small);
+                      (fusion == GPR_FUSION_ON || fuse_small);
     if (fuse) {
         GPR_HIP(hipEventRecord(g->ev0, g->stream));
         if (in_kind != GPR_FRONTIER_LIST) GPR_TRY(frontier_to_list(in));
@@ -557,7 +672,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         GPR_HIP(hipEventSynchronize(g->ev1));
         float ms = 0.f;
         GPR_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
-        if (use_pol) policy_after_step(g, nacte, 0);
+        if (use_pol) policy_after_step(g, nacte, -1);   // no inference in the fused kernel: the host's next
         s.direction = GPR_DIR_PUSH;
         s.out_kind = out ? out->kind : GPR_FRONTIER_EMPTY;
         s.fused = 1;
@@ -611,7 +726,8 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         int record = out ? desc->record : GPR_RECORD_NONE;
         ProfScope ps(g, "pull");
 #define PULL(R)                                                                                                 \
-    hipLaunchKernelGGL(k_pull<R>, dim3(pull_grid(n)), dim3(PULL_THREADS), 0, g->stream, n, g->cindex, g->cnbrs, g->deg, \
+    hipLaunchKernelGGL(k_pull<R>, dim3(pull_grid(n)), dim3(PULL_THREADS), ol ? PULL_QBYTES : 0, g->stream, n,           \
+                       g->cindex, g->cnbrs, g->deg,                                                              \
                        prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol, g->ctr, pol,            \
                        (uint32_t)tun().pull_serial_max)
         if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1);
@@ -638,7 +754,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         s.nacte_out = out->nacte;
     }
     if (dir == GPR_DIR_PULL) s.edges_scanned = g->hctr->edges;
-    if (use_pol) policy_after_step(g, nacte, g->hctr->decision);
+    if (use_pol) policy_after_step(g, nacte, forced ? -1 : (int)g->hctr->decision);
     s.direction = dir;
     s.out_kind = okind;
     s.ms = ms;
@@ -736,12 +852,21 @@ extern "C" int gpr_vertex_filter(gpr_graph *g, gpr_frontier *in, gpr_prop *p, ui
     }
     if (kind == GPR_FRONTIER_LIST) {
         GPR_TRY(frontier_ensure_list(out));
-        hipLaunchKernelGGL(k_vfilter_list, dim3(grid_for(in->nactv, 256, 4096)), dim3(256), 0, g->stream, in->list,
-                           in->nactv, (const uint32_t *)p->d, value, neg, out->list, g->deg, g->ctr);
+        const uint32_t k = in->nactv;
+        const unsigned blocks = (unsigned)(((uint64_t)k + VF_TILE - 1) / VF_TILE);
+        GPR_TRY(ensure_partials(g, blocks + 1));
+        unsigned long long *cnt = (unsigned long long *)g->partial;
+        if (blocks) {
+            hipLaunchKernelGGL(k_vfilter_list, dim3(blocks), dim3(VF_THREADS), 0, g->stream, in->list, k,
+                               (const uint32_t *)p->d, value, neg, out->list, g->deg, cnt, 0, g->ctr);
+            excl_scan_counts(g, cnt, blocks);
+            hipLaunchKernelGGL(k_vfilter_list, dim3(blocks), dim3(VF_THREADS), 0, g->stream, in->list, k,
+                               (const uint32_t *)p->d, value, neg, out->list, g->deg, cnt, 1, g->ctr);
+        }
         GPR_HIP(hipGetLastError());
         GPR_TRY(read_counters(g));
         out->kind = GPR_FRONTIER_LIST;
-        out->nactv = g->hctr->listpos;
+        out->nactv = g->hctr->nactv;
         out->nacte = g->hctr->nacte;
     } else {
         GPR_TRY(frontier_ensure_bits(out));

@@ -68,8 +68,8 @@ k_bits_to_list(const uint64_t *__restrict__ bits, size_t words, uint32_t *list,
         uint64_t etot = block_sum(es, redl);
         if (threadIdx.x == 0) {
             blk_counts[blockIdx.x] = tot;
-            atomicAdd(&ctr->nactv, tot);
-            atomicAdd(&ctr->nacte, (unsigned long long)etot);
+            atomicAdd(&ctr_stripe(ctr)->nactv, tot);
+            atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)etot);
         }
         return;
     }
@@ -129,7 +129,7 @@ __global__ void k_list_edges(const uint32_t *__restrict__ list, uint32_t k,
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < k; i += gridDim.x * blockDim.x)
         s += deg[list[i]];
     s = block_sum(s, red);
-    if (threadIdx.x == 0 && s) atomicAdd(&ctr->nacte, (unsigned long long)s);
+    if (threadIdx.x == 0 && s) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)s);
 }
 
 int fill_words(gpr_graph *g, uint64_t *p, size_t words, uint64_t v) {
@@ -192,6 +192,10 @@ int frontier_to_bits(gpr_frontier *f) {
     GPR_HIP(hipGetLastError());
     f->kind = GPR_FRONTIER_BITMAP;
     return GPR_OK;
+}
+
+void excl_scan_counts(gpr_graph *g, unsigned long long *counts, uint32_t count) {
+    hipLaunchKernelGGL(k_excl_scan_u64, dim3(1), dim3(1024), 0, g->stream, counts, count);
 }
 
 // Bitmap/all -> ascending list.  Recomputes nactv and nacte.

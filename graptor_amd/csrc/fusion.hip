@@ -90,12 +90,18 @@ __device__ __forceinline__ uint32_t slices_of(uint32_t dg) {
 }
 
 // Append the destinations d[j] with push[j] set to the next lists (light or
-// heavy by degree dg[j]).  Wave-uniform: every lane calls it.
+// heavy by degree dg[j]).  Wave-uniform: every lane calls it.  With
+// dense_at set, a wave that has long rows to list first adds its edges to
+// the level's count; once that count is above dense_at the level will be
+// handed back unprocessed (the check at the next barrier reads the same,
+// only growing, count), so its long rows are listed whole, as light
+// entries, instead of one entry per slice: a hub-heavy level that ends the
+// step costs one entry per vertex, not one per FU_SLICE edges.
 __device__ __forceinline__ void append_next(const bool (&push)[FU_U], const uint32_t (&d)[FU_U],
                                             const uint32_t (&dg)[FU_U], const FusedLists &L, int nb,
                                             unsigned long long *nlight, unsigned long long *nheavy,
-                                            unsigned long long *nedge) {
-    uint32_t cl = 0, ch = 0;
+                                            unsigned long long *nedge, unsigned long long dense_at) {
+    uint32_t cl = 0, ch = 0, cv = 0;
     unsigned long long ce = 0;
 #pragma unroll
     for (int j = 0; j < FU_U; ++j) {
@@ -103,25 +109,41 @@ __device__ __forceinline__ void append_next(const bool (&push)[FU_U], const uint
             const uint32_t s = slices_of(dg[j]);
             cl += s == 0;
             ch += s;
+            cv += 1;
             ce += dg[j] + 1ull;
         }
     }
     if (!__ballot(cl | ch)) return;
+    ce = wave_sum(ce);
+    const uint32_t th0 = wave_sum(ch);
+    int whole = 0;   // list long rows whole: the level is already dense
+    if (lane() == 0) {
+        if (th0 && dense_at) {
+            const unsigned long long before =
+                __hip_atomic_fetch_add(nedge, ce, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            whole = before + ce > dense_at;
+        } else {
+            __hip_atomic_fetch_add(nedge, ce, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    whole = __shfl(whole, 0, 64);
+    if (whole) {
+        cl = cv;
+        ch = 0;
+    }
     const uint32_t il = wave_incl_scan(cl), ih = wave_incl_scan(ch);
     const uint32_t tl = __shfl(il, 63, 64), th = __shfl(ih, 63, 64);
-    ce = wave_sum(ce);
     unsigned long long bl = 0, bh = 0;
     if (lane() == 0) {
         if (tl) bl = __hip_atomic_fetch_add(nlight, (unsigned long long)tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (th) bh = __hip_atomic_fetch_add(nheavy, (unsigned long long)th, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(nedge, ce, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     bl = bcast64(bl) + (il - cl);
     bh = bcast64(bh) + (ih - ch);
 #pragma unroll
     for (int j = 0; j < FU_U; ++j) {
         if (push[j]) {
-            const uint32_t s = slices_of(dg[j]);
+            const uint32_t s = whole ? 0u : slices_of(dg[j]);
             if (s == 0) {
                 __hip_atomic_store(&L.light[nb][bl++], d[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             } else {
@@ -147,7 +169,7 @@ template <int RELAX>
 __device__ __forceinline__ void relax_edges(const bool (&act)[FU_U], const uint32_t (&d)[FU_U], uint32_t val,
                                             uint32_t *prop, const uint32_t *__restrict__ deg, const FusedLists &L,
                                             int nb, unsigned long long *nlight, unsigned long long *nheavy,
-                                            unsigned long long *nedge) {
+                                            unsigned long long *nedge, unsigned long long dense_at) {
     bool need[FU_U];
     bool any_need = false;
 #pragma unroll
@@ -188,7 +210,7 @@ __device__ __forceinline__ void relax_edges(const bool (&act)[FU_U], const uint3
     bool push[FU_U];
 #pragma unroll
     for (int j = 0; j < FU_U; ++j) push[j] = low[j] && !((ow[j] >> (d[j] & 63)) & 1ull);
-    if (__ballot(any)) append_next(push, d, dg, L, nb, nlight, nheavy, nedge);
+    if (__ballot(any)) append_next(push, d, dg, L, nb, nlight, nheavy, nedge, dense_at);
 }
 
 template <int RELAX>
@@ -255,7 +277,7 @@ k_fused(const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs, c
                     edges += act[j];
                 }
                 e = e + FU_U < e1 ? e + FU_U : e1;
-                relax_edges<RELAX>(act, d, val, prop, deg, L, nb, nlight, nheavy, nedge);
+                relax_edges<RELAX>(act, d, val, prop, deg, L, nb, nlight, nheavy, nedge, dense_at);
             }
         }
         // ---- heavy rows: wave per slice, waves from the back
@@ -281,15 +303,19 @@ k_fused(const uint64_t *__restrict__ index, const uint32_t *__restrict__ nbrs, c
                 d[k] = nbrs[act[k] ? e : act[0] ? e0 + ln : e0];   // clamped: the load is unconditional
                 edges += act[k];
             }
-            relax_edges<RELAX>(act, d, val, prop, deg, L, nb, nlight, nheavy, nedge);
+            relax_edges<RELAX>(act, d, val, prop, deg, L, nb, nlight, nheavy, nedge, dense_at);
         }
         if (!grid_barrier(&ctl->arrive, &ctl->error, ctl->group, lvl, spin_limit, &s_ok)) break;
     }
-    edges = wave_sum(edges);
-    pops = wave_sum(pops);
-    if (ln == 0) {
-        atomicAdd(&ctl->edges, edges);
-        atomicAdd(&ctl->pops, pops);
+    // one pair of atomics per block, none for a block without work: a
+    // small step leaves most of the 4096 waves idle, and an atomic from
+    // each of them on one line cost more than the step's work
+    __shared__ unsigned long long red[FU_WAVES];
+    edges = block_sum(edges, red);
+    pops = block_sum(pops, red);
+    if (threadIdx.x == 0) {
+        if (edges) atomicAdd(&ctl->edges, edges);
+        if (pops) atomicAdd(&ctl->pops, pops);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) ctl->levels = lvl;
 }
@@ -320,7 +346,8 @@ __global__ void k_fused_seed(const uint32_t *__restrict__ list, uint32_t k, cons
             dg[0] = deg[d[0]];
             push[0] = dg[0] > FU_LIGHT_MAX;
         }
-        if (__ballot(push[0])) append_next(push, d, dg, L, 0, &ctl->nlight[0], &ctl->nheavy[0], &ctl->nedge[0]);
+        if (__ballot(push[0]))
+            append_next(push, d, dg, L, 0, &ctl->nlight[0], &ctl->nheavy[0], &ctl->nedge[0], 0ull);
     }
 }
 
@@ -444,7 +471,10 @@ int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_fr
         const uint32_t *in_list = in->list;
         void *args[] = {(void *)&index, (void *)&nbrs, (void *)&deg, (void *)&prop, (void *)&L, (void *)&in_list,
                         (void *)&k, (void *)&ctl, (void *)&spin, (void *)&dense};
-        GPR_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(FU_THREADS), args, 0, g->stream));
+        if (tun().fusion_cooperative >= 0.5)
+            GPR_HIP(hipLaunchCooperativeKernel(kern, dim3(nblocks), dim3(FU_THREADS), args, 0, g->stream));
+        else
+            GPR_HIP(hipLaunchKernel(kern, dim3(nblocks), dim3(FU_THREADS), args, 0, g->stream));
     }
     // the first line of the control block: counts and flags
     FusedCtl h;
@@ -459,6 +489,7 @@ int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_fr
         return GPR_E_HIP;
     }
     s->edges_scanned = h.edges;
+    s->levels = (int)h.levels;
     s->nactv_out = 0;
     s->nacte_out = 0;
     if (out) {

@@ -93,9 +93,13 @@ static int finish_graph(gpr_graph *g, const uint64_t *host_index, const uint64_t
     GPR_TRY(build_partitions(g, host_cindex));
     g->hdv = highest_degree(host_index, n);
     g->max_deg = n ? (uint32_t)(host_index[g->hdv + 1] - host_index[g->hdv]) : 0;
-    GPR_TRY(dev_alloc((void **)&g->ctr, sizeof(DevCounters), "counters"));
-    GPR_HIP(hipMemset(g->ctr, 0, sizeof(DevCounters)));
-    GPR_HIP(hipHostMalloc((void **)&g->hctr, sizeof(DevCounters), hipHostMallocDefault));
+    GPR_TRY(dev_alloc((void **)&g->ctr, COUNTER_BYTES, "counters"));
+    GPR_HIP(hipMemset(g->ctr, 0, COUNTER_BYTES));
+    // the counter read-back target: fine-grained (coherent) pinned host
+    // memory that a one-wave kernel writes the sums into, so that a step's
+    // counts come back with the stream's completion and no copy
+    GPR_HIP(hipHostMalloc((void **)&g->hctr, sizeof(DevCounters), hipHostMallocMapped | hipHostMallocCoherent));
+    GPR_HIP(hipHostGetDevicePointer((void **)&g->hctr_dev, g->hctr, 0));
     GPR_TRY(dev_alloc((void **)&g->dedup, sizeof(uint64_t) * nwords(n), "dedup bitmap"));
     GPR_HIP(hipMemset(g->dedup, 0, sizeof(uint64_t) * nwords(n)));
     GPR_TRY(dev_alloc((void **)&g->scan, sizeof(uint64_t) * ((size_t)n + 1), "frontier prefix"));

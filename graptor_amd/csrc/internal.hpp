@@ -68,22 +68,40 @@ struct Tunables {
     double fusion_spin_limit = 1 << 22;       // fused push: polls without progress before it gives up
     double fusion_blocks = 0;                 // fused push: cooperative grid size (0: one per CU)
     double pr_write_stage_cells = 4096;       // PR write: a tile's run offsets go to LDS up to this many cells
-    double fusion_small_limit = 1 << 17;      // GPR_FUSION_SMALL: frontier size that fuses / level size that ends it
+    double fusion_small_limit = 1 << 17;      // GPR_FUSION_SMALL / _DEFAULT: frontier size that fuses / level size that ends it
+    double fusion_cooperative = 1;            // fused push: 1 = hipLaunchCooperativeKernel, 0 = a plain launch
 };
 Tunables &tun();
 int num_cus();
 
 // ---- device-resident counters of one handle ------------------------------
-struct DevCounters {
-    unsigned long long nacte;   // edges of the frontier being produced
-    unsigned long long edges;   // edges examined
-    unsigned int nactv;         // vertices of the frontier being produced
+// The frontier counts are striped: workgroup b adds its totals to stripe
+// b % CTR_STRIPES (one 128-byte line each, right after the DevCounters
+// line), so no single line takes an atomic from every workgroup of a large
+// grid; the host sums the stripes when it reads the counters back, and
+// device readers sum them too.  The list append position stays one word
+// (it allocates positions).
+constexpr int CTR_STRIPES = 64;
+struct alignas(128) DevCounters {
+    unsigned long long nacte;   // edges of the frontier being produced (host: sum of the stripes)
+    unsigned long long edges;   // edges examined (host: sum of the stripes)
+    unsigned int nactv;         // vertices of the frontier being produced (host: sum of the stripes)
     unsigned int listpos;       // append position of the list being produced
     unsigned int ticket;        // last-workgroup tickets (reset by that wg)
     unsigned int ticket2;
     unsigned int decision;      // policy choice for the next step
     unsigned int pad;
 };
+struct alignas(128) CtrStripe {
+    unsigned long long nacte;
+    unsigned long long edges;
+    unsigned int nactv;
+    unsigned int ticket;        // workgroups of this stripe done (reset by its last)
+};
+constexpr size_t COUNTER_BYTES = sizeof(DevCounters) + CTR_STRIPES * sizeof(CtrStripe);
+__device__ __forceinline__ CtrStripe *ctr_stripe(DevCounters *c) {
+    return reinterpret_cast<CtrStripe *>(c + 1) + (blockIdx.x % CTR_STRIPES);
+}
 
 // Scalars of the PageRank iteration, kept on the device so that an
 // iteration needs no host round trip.
@@ -178,7 +196,8 @@ struct gpr_graph {
     hipStream_t stream = nullptr;
     // scratch
     gpr::DevCounters *ctr = nullptr;     // device
-    gpr::DevCounters *hctr = nullptr;    // pinned host mirror
+    gpr::DevCounters *hctr = nullptr;    // host mirror: fine-grained pinned memory the device writes
+    gpr::DevCounters *hctr_dev = nullptr;  // its device address
     uint64_t *dedup = nullptr;           // n/64 words, all zero between steps
     uint64_t *scan = nullptr;            // frontier degree prefix (n+1)
     uint64_t *scan_blk = nullptr;        // per-scan-block totals
@@ -197,11 +216,19 @@ struct gpr_graph {
     std::vector<void *> pool_list, pool_bits;
     // learned policy state of the current run
     float *pol_params = nullptr;    // device copy of the network (null: none)
+    float pol_margin = 0.f;         // gate margin loaded with the network
     std::vector<float> pol_host;    // host copy (first-step decision)
     bool pol_have = false;          // a device decision for the next step exists
     int pol_next = 0;
     int pol_iter = 0;
     double pol_unvisited = 0;
+    // trace runs: step pol_force_step takes action pol_force_action and every
+    // other step the rule's; with pol_log set, each step's features are logged
+    int pol_force_step = -1;
+    int pol_force_action = 0;
+    double pol_force_t0 = -1.0;   // host clock (ms) when the forced step began
+    bool pol_log = false;
+    std::vector<float> pol_feat_log;
     uint32_t max_deg = 0;
     gpr::ColdLayout *cold = nullptr;   // PageRank cold-edge layout (built on first use)
     void *fbuf = nullptr;              // fused push: level lists and second bitmap (built on first use)
@@ -260,6 +287,8 @@ void prof_flush(gpr_graph *g);
 
 // Fused sparse push (fusion.hip): process the frontier and everything it
 // activates in one launch; `in` must be a list.
+// In-place exclusive scan of per-workgroup counts (one workgroup, on g's stream).
+void excl_scan_counts(gpr_graph *g, unsigned long long *counts, uint32_t count);
 int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_frontier *out, uint64_t dense_at,
                gpr_iter_stats *s);
 

@@ -199,7 +199,7 @@ k_ms_finalize(uint32_t n, uint32_t level, uint32_t *levels, uint64_t *seen, uint
         }
     }
     const uint64_t te = block_sum(my_e, red);
-    if (threadIdx.x == 0 && te) atomicAdd(&ctr->nacte, (unsigned long long)te);
+    if (threadIdx.x == 0 && te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
 }
 
 // edges traversed: sum over sources of the out-degrees of reached vertices
@@ -210,7 +210,7 @@ k_ms_edges(uint32_t n, const uint64_t *__restrict__ seen, const uint32_t *__rest
     for (uint32_t v = blockIdx.x * 256 + threadIdx.x; v < n; v += gridDim.x * 256)
         s += (uint64_t)__popcll(seen[v]) * deg[v];
     s = block_sum(s, red);
-    if (threadIdx.x == 0 && s) atomicAdd(&ctr->edges, (unsigned long long)s);
+    if (threadIdx.x == 0 && s) atomicAdd(&ctr_stripe(ctr)->edges, (unsigned long long)s);
 }
 
 __global__ void k_ms_scatter(const uint32_t *__restrict__ src, const uint32_t *__restrict__ order, uint32_t n,

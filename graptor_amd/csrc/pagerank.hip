@@ -24,6 +24,7 @@
 #include "internal.hpp"
 #include "device.hpp"
 #include "pr_codec.hpp"
+#include "grid_sync.hpp"
 
 #include <math.h>
 
@@ -244,16 +245,18 @@ k_pr_reduce(const double *__restrict__ part, const float *__restrict__ part_max,
         }
         __syncthreads();
     }
+    // slices: agent-scope stores, acknowledged (s_waitcnt vmcnt(0)) before
+    // the ticket; the last workgroup reads them with agent-scope loads (no
+    // fence, which would write back the XCD's L2)
     if (threadIdx.x == 0) {
-        slices[3 * blockIdx.x] = r1[0];
-        slices[3 * blockIdx.x + 1] = r2[0];
-        slices[3 * blockIdx.x + 2] = (double)r3[0];
-        __threadfence();
+        __hip_atomic_store(&slices[3 * blockIdx.x], r1[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slices[3 * blockIdx.x + 1], r2[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&slices[3 * blockIdx.x + 2], (double)r3[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         last = atomicAdd(ticket, 1u) == gridDim.x - 1;
     }
     __syncthreads();
     if (!last || threadIdx.x != 0) return;
-    __threadfence();
     double a = 0.0, c = 0.0, m = 0.0;
     for (unsigned i = 0; i < gridDim.x; ++i) {
         a += __hip_atomic_load(&slices[3 * i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

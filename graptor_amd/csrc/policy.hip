@@ -1,20 +1,25 @@
-// policy.hip -- the per-iteration choice of traversal direction and output
-// frontier representation.
+// policy.hip -- the per-iteration choice of traversal direction, output
+// frontier representation and fusion.
 //
 // Fallback and judge: the reference's threshold rule (sparse push iff
 // nactv + nacte <= m/20, api.h:150-153; a list output for push and a dense
-// output for pull, dsl/edgemap.h:1571-1601).  Learned policy: a 6-32-32-4
-// MLP (policy_dev.hpp) whose forward pass runs in the last workgroup of each
-// push/pull kernel; the host reads its choice together with the frontier
-// counts it already reads.  Outputs are identical whichever choice is made
-// (the relax is a min, the frontier a set); only the time differs.
+// output for pull, dsl/edgemap.h:1571-1601) with the benchmark's own
+// fusion setting.  Learned policy: a 6-32-32-4 MLP (policy_dev.hpp) whose
+// forward pass runs in the last workgroup of each push/pull kernel; the
+// host reads its choice together with the frontier counts it already
+// reads (after a fused step, whose kernel does not run it, the host
+// evaluates it).  Its four actions are the rule's own step, an unfused
+// push, a push fused while its levels stay small, and a pull with a list
+// output; another action replaces the rule's only with a logit lead of at
+// least the loaded margin.  Outputs are identical whichever action is
+// taken (the relax is a min, the frontier a set); only the time differs.
 //
-// Training data: gpr_policy_trace runs BFS or CC and, at every iteration,
-// times all four choices from a snapshot of the state, each followed by the
-// better of push/pull for the next step (so the cost of the representation
-// handed to the next step counts), then advances with the fastest choice.
-// Training: full-batch Adam on the GPU with this file's forward/backward
-// kernels (per-sample soft targets and weights from the timings).
+// Training data: gpr_policy_trace runs the BFSLVLv or CCv benchmark once
+// under the rule (logging each step's features), then again with one step
+// forced to each action, at evenly spread steps; the time from the start
+// of that step to the end of the run says what each action costs.  Training:
+// full-batch Adam on the GPU with this file's forward/backward kernels
+// (per-sample soft targets and weights from the timings).
 #include "internal.hpp"
 #include "device.hpp"
 #include "policy_dev.hpp"
@@ -25,6 +30,7 @@
 #include <cmath>
 
 #include <algorithm>
+#include <chrono>
 #include <vector>
 
 namespace gpr {
@@ -32,15 +38,16 @@ namespace gpr {
 // ---- host mirror of the feature map and the forward pass --------------------
 static void host_features(const gpr_graph *g, uint32_t nactv, uint64_t nacte, double unvisited, int iter, float f[6]) {
     const double mean = g->n ? (double)g->m / (double)g->n : 0.0;
+    const float lm = log2f(1.0f + (float)g->m);
     f[0] = log2f(1.0f + (float)nactv) / 32.0f;
-    f[1] = log2f(1.0f + (float)nacte) / 40.0f;
-    f[2] = (float)log2(1.0 + (unvisited > 0 ? unvisited : 0.0)) / 40.0f;
+    f[1] = 1.0f + (log2f(1.0f + (float)nacte) - lm) / 32.0f;
+    f[2] = 1.0f + ((float)log2(1.0 + (unvisited > 0 ? unvisited : 0.0)) - lm) / 32.0f;
     f[3] = (float)iter / 64.0f;
     f[4] = (float)log2(1.0 + mean) / 8.0f;
     f[5] = (float)log2(1.0 + (mean > 0 ? g->max_deg / mean : 0.0)) / 24.0f;
 }
 
-static int host_forward(const float *p, const float f[6]) {
+static int host_forward(const float *p, float margin, const float f[6]) {
     const MlpView P(p);
     float h1[MLP_H], h2[MLP_H], z[MLP_OUT];
     for (int j = 0; j < MLP_H; ++j) {
@@ -53,101 +60,57 @@ static int host_forward(const float *p, const float f[6]) {
         for (int i = 0; i < MLP_H; ++i) a += h1[i] * P.w2[i * MLP_H + j];
         h2[j] = a > 0.f ? a : 0.f;
     }
-    int best = 0;
     for (int k = 0; k < MLP_OUT; ++k) {
         float a = P.b3[k];
         for (int i = 0; i < MLP_H; ++i) a += h2[i] * P.w3[i * MLP_OUT + k];
         z[k] = a;
-        if (z[k] > z[best]) best = k;
     }
-    return best;
+    return policy_gate(z, margin);
 }
 
 PolicyCtx policy_ctx(gpr_graph *g, uint64_t nacte_in) {
     PolicyCtx c = {};
     c.params = g->pol_params;
+    c.margin = g->pol_margin;
     const double mean = g->n ? (double)g->m / (double)g->n : 0.0;
     c.mean_deg_feat = (float)log2(1.0 + mean) / 8.0f;
     c.skew_feat = (float)log2(1.0 + (mean > 0 ? g->max_deg / mean : 0.0)) / 24.0f;
     c.iter_feat = (float)(g->pol_iter + 1) / 64.0f;
+    c.log_m = log2f(1.0f + (float)g->m);
     c.unvisited_before = g->pol_unvisited - (double)nacte_in;
     return c;
 }
 
-// Choice for the current step.  No network loaded: the threshold rule.
-int policy_decide(gpr_graph *g, uint32_t nactv, uint64_t nacte, int *dir, int *out_kind) {
-    int d;
-    if (!g->pol_params) {
-        const bool sparse = (uint64_t)nactv + nacte <= g->m / 20;
-        *dir = sparse ? GPR_DIR_PUSH : GPR_DIR_PULL;
-        if (*out_kind == 0) *out_kind = sparse ? GPR_FRONTIER_LIST : GPR_FRONTIER_BITMAP;
-        return GPR_OK;
-    }
-    if (g->pol_have) {
-        d = g->pol_next;
-    } else {
-        float f[6];
-        host_features(g, nactv, nacte, g->pol_unvisited - (double)nacte, g->pol_iter, f);
-        d = host_forward(g->pol_host.data(), f);
-    }
-    *dir = d < 2 ? GPR_DIR_PUSH : GPR_DIR_PULL;
-    *out_kind = (d & 1) ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST;
-    return GPR_OK;
+// The action of the current step (a network is loaded): the choice the
+// device made at the end of the previous step, or the host's own for the
+// first step and after a fused step.
+int policy_action(gpr_graph *g, uint32_t nactv, uint64_t nacte) {
+    if (g->pol_have) return g->pol_next;
+    float f[6];
+    host_features(g, nactv, nacte, g->pol_unvisited - (double)nacte, g->pol_iter, f);
+    return host_forward(g->pol_host.data(), g->pol_margin, f);
 }
 
-// Book-keeping after a step run under the policy.
-void policy_after_step(gpr_graph *g, uint64_t nacte_in, uint32_t decision) {
+// Monotonic host clock in milliseconds.
+double host_clock_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Trace runs: the features of the step about to run.
+void policy_log_features(gpr_graph *g, uint32_t nactv, uint64_t nacte) {
+    float f[6];
+    host_features(g, nactv, nacte, g->pol_unvisited - (double)nacte, g->pol_iter, f);
+    g->pol_feat_log.insert(g->pol_feat_log.end(), f, f + 6);
+}
+
+// Book-keeping after a step run under the policy; decision < 0: none was
+// made on the device.
+void policy_after_step(gpr_graph *g, uint64_t nacte_in, int decision) {
     g->pol_unvisited -= (double)nacte_in;
     if (g->pol_unvisited < 0) g->pol_unvisited = 0;
     g->pol_iter++;
-    g->pol_next = (int)decision;
-    g->pol_have = true;
-}
-
-// ---- frontier snapshots ------------------------------------------------------------
-struct FrontierCopy {
-    int kind = GPR_FRONTIER_EMPTY;
-    uint32_t nactv = 0;
-    uint64_t nacte = 0;
-    uint32_t *list = nullptr;
-    uint64_t *bits = nullptr;
-};
-
-static int save_frontier(gpr_graph *g, const gpr_frontier *f, FrontierCopy &c) {
-    c.kind = f->kind;
-    c.nactv = f->nactv;
-    c.nacte = f->nacte;
-    if (f->kind == GPR_FRONTIER_LIST && f->nactv) {
-        if (!c.list) GPR_TRY(dev_alloc((void **)&c.list, 4ull * std::max<uint32_t>(g->n, 1), "snapshot"));
-        GPR_HIP(hipMemcpyAsync(c.list, f->list, 4ull * f->nactv, hipMemcpyDeviceToDevice, g->stream));
-    }
-    if (f->kind == GPR_FRONTIER_BITMAP) {
-        if (!c.bits) GPR_TRY(dev_alloc((void **)&c.bits, 8ull * nwords(g->n), "snapshot"));
-        GPR_HIP(hipMemcpyAsync(c.bits, f->bits, 8ull * nwords(g->n), hipMemcpyDeviceToDevice, g->stream));
-    }
-    return GPR_OK;
-}
-
-static int restore_frontier(gpr_graph *g, gpr_frontier *f, const FrontierCopy &c) {
-    f->kind = c.kind;
-    f->nactv = c.nactv;
-    f->nacte = c.nacte;
-    if (c.kind == GPR_FRONTIER_LIST && c.nactv) {
-        GPR_TRY(frontier_ensure_list(f));
-        GPR_HIP(hipMemcpyAsync(f->list, c.list, 4ull * c.nactv, hipMemcpyDeviceToDevice, g->stream));
-    }
-    if (c.kind == GPR_FRONTIER_BITMAP) {
-        GPR_TRY(frontier_ensure_bits(f));
-        GPR_HIP(hipMemcpyAsync(f->bits, c.bits, 8ull * nwords(g->n), hipMemcpyDeviceToDevice, g->stream));
-    }
-    return GPR_OK;
-}
-
-static void free_copy(FrontierCopy &c) {
-    dev_free(c.list);
-    dev_free(c.bits);
-    c.list = nullptr;
-    c.bits = nullptr;
+    g->pol_next = decision < 0 ? 0 : decision;
+    g->pol_have = decision >= 0;
 }
 
 // ---- MLP training kernels ------------------------------------------------------------
@@ -325,13 +288,18 @@ extern "C" int gpr_policy_set(gpr_graph *g, const float *params, int count) {
         dev_free(g->pol_params);
         g->pol_params = nullptr;
         g->pol_host.clear();
+        g->pol_margin = 0.f;
         return GPR_OK;
     }
-    GPR_REQUIRE(count == MLP_PARAMS, "policy: expected %d parameters, got %d", MLP_PARAMS, count);
+    GPR_REQUIRE(count == MLP_PARAMS || count == MLP_PARAMS + 1,
+                "policy: expected %d parameters (or one more, the gate margin), got %d", MLP_PARAMS, count);
     for (int i = 0; i < count; ++i) GPR_REQUIRE(std::isfinite(params[i]), "policy: parameter %d is not finite", i);
+    const float margin = count > MLP_PARAMS ? params[MLP_PARAMS] : 0.f;
+    GPR_REQUIRE(margin >= 0.f, "policy: the gate margin must not be negative");
     if (!g->pol_params) GPR_TRY(dev_alloc((void **)&g->pol_params, 4ull * MLP_PARAMS, "policy"));
     GPR_HIP(hipMemcpy(g->pol_params, params, 4ull * MLP_PARAMS, hipMemcpyHostToDevice));
     g->pol_host.assign(params, params + MLP_PARAMS);
+    g->pol_margin = margin;
     return GPR_OK;
 }
 
@@ -412,109 +380,76 @@ extern "C" int gpr_mlp_train(float *params, const float *x, const float *targets
     return GPR_OK;
 }
 
-// Trace generation: algo 0 = BFSLVLv from `start`, 1 = CCv (all-true start,
-// no PRESET_ZERO).  features: k x 6, times: k x 4 (ms of the step plus the
-// better next step).
+// Trace generation: the benchmarks themselves (algo 0 = BFSLVLv from
+// `start`, 1 = CCv), each with its default settings.  A first run under
+// the rule logs every step's features and counts the steps; then, at up to
+// cap evenly spread steps i, the run is repeated with step i forced to each
+// action and every other step the rule's.  The runs share the steps before
+// i, so times[k][a] (the best of two runs' device time of run(), host round
+// trips included) differ by what action a at step i costs to the end.
+extern "C" int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *o, uint32_t *levels, gpr_run_info *info);
+extern "C" int gpr_ccv(gpr_graph *g, const gpr_run_opts *o, uint32_t *labels, gpr_run_info *info);
+
 extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap, float *features, float *times,
                                 int *k) {
     if (!g || !k || cap <= 0 || (algo != 0 && algo != 1)) return GPR_E_INVAL;
     GPR_REQUIRE(algo == 1 || start < g->n, "trace: start out of range");
-    const uint32_t n = g->n;
-    gpr_prop *P = nullptr, *Pb = nullptr, *Pb2 = nullptr;
-    gpr_frontier *F = nullptr, *O = nullptr, *O2 = nullptr;
-    FrontierCopy fs, os;
+    *k = 0;
     float *saved = g->pol_params;
-    g->pol_params = nullptr;   // the steps below force their choices
-    int cnt = 0;
-    int r = GPR_OK;
-    auto step = [&](gpr_frontier *in, gpr_frontier *out, int choice, int iter, gpr_iter_stats *st) -> int {
-        gpr_emap_desc d = {};
-        d.relax = algo == 0 ? GPR_RELAX_MIN_PLUS1 : GPR_RELAX_MIN;
-        d.prop = P;
-        d.dst_filter = algo == 0 ? GPR_DST_GT : GPR_DST_NE;
-        d.dst_arg = algo == 0 ? (uint32_t)iter + 1 : 0u;
-        d.record = GPR_RECORD_REDUCTION;
-        d.direction = choice < 2 ? GPR_DIR_PUSH : GPR_DIR_PULL;
-        d.out_kind = (choice & 1) ? GPR_FRONTIER_BITMAP : GPR_FRONTIER_LIST;
-        d.threshold_pct = 5.f;
-        return gpr_edgemap(g, &d, in, out, st);
-    };
-    do {
-        if ((r = gpr_prop_create(g, GPR_U32, &P))) break;
-        if ((r = gpr_prop_create(g, GPR_U32, &Pb))) break;
-        if ((r = gpr_prop_create(g, GPR_U32, &Pb2))) break;
-        if ((r = gpr_frontier_create(g, &F))) break;
-        if ((r = gpr_frontier_create(g, &O))) break;
-        if ((r = gpr_frontier_create(g, &O2))) break;
-        if (algo == 0) {
-            if ((r = gpr_vertex_map(g, GPR_VMAP_FILL, P, nullptr, nullptr, n + 1))) break;
-            if ((r = gpr_frontier_set_single(F, start))) break;
-            if ((r = gpr_vertex_map(g, GPR_VMAP_FILL_FRONTIER, P, nullptr, F, 0))) break;
-        } else {
-            if ((r = gpr_vertex_map(g, GPR_VMAP_IOTA, P, nullptr, nullptr, 0))) break;
-            if ((r = gpr_frontier_set_all(F))) break;
+    g->pol_params = nullptr;
+    gpr_run_opts o = {};
+    o.direction = GPR_DIR_POLICY;
+    o.max_iter = 100;
+    o.tol = 1e-7;
+    // one run with step `step` forced to `action`; the best of `reps` times
+    // from the start of that step to the end of the run (host clock: what
+    // the step and the steps after it cost, launches and waits included,
+    // without the noise of the steps before it), or of the whole run when
+    // the run ends before that step
+    auto run = [&](int step, int action, int reps, double *ms) -> int {
+        g->pol_force_step = step;
+        g->pol_force_action = action;
+        *ms = 1e30;
+        for (int rep = 0; rep < reps; ++rep) {
+            gpr_run_info info = {};
+            g->pol_force_t0 = -1.0;
+            const double t_start = host_clock_ms();
+            const int rc = algo == 0 ? gpr_bfslvl(g, start, &o, nullptr, &info) : gpr_ccv(g, &o, nullptr, &info);
+            if (rc != GPR_OK) return rc;
+            const double t_end = host_clock_ms();
+            *ms = std::min(*ms, t_end - (g->pol_force_t0 >= 0 ? g->pol_force_t0 : t_start));
         }
-        double U = (double)g->m;
-        int iter = 0;
-        while (F->nactv > 0 && F->kind != GPR_FRONTIER_EMPTY && cnt < cap) {
-            float f[6];
-            host_features(g, F->nactv, F->nacte, U - (double)F->nacte, iter, f);
-            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, Pb, P, nullptr, 0))) break;
-            if ((r = save_frontier(g, F, fs))) break;
-            double best = 1e30;
-            int bestc = 0;
-            for (int c = 0; c < 4 && !r; ++c) {
-                double tc = 1e30;
-                for (int rep = 0; rep < 2 && !r; ++rep) {
-                    if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb, nullptr, 0))) break;
-                    if ((r = restore_frontier(g, F, fs))) break;
-                    gpr_iter_stats st = {};
-                    if ((r = step(F, O, c, iter, &st))) break;
-                    double t = st.ms;
-                    if (O->nactv > 0 && O->kind != GPR_FRONTIER_EMPTY) {
-                        // lookahead: the better direction for the next step
-                        if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, Pb2, P, nullptr, 0))) break;
-                        if ((r = save_frontier(g, O, os))) break;
-                        double t2 = 1e30;
-                        for (int c2 : {0, 3}) {
-                            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb2, nullptr, 0))) break;
-                            if ((r = restore_frontier(g, O, os))) break;
-                            gpr_iter_stats st2 = {};
-                            if ((r = step(O, O2, c2, iter + 1, &st2))) break;
-                            t2 = std::min(t2, st2.ms);
-                        }
-                        t += t2;
-                    }
-                    tc = std::min(tc, t);
-                }
-                if (times) times[cnt * 4 + c] = (float)tc;
-                if (tc < best) {
-                    best = tc;
-                    bestc = c;
-                }
+        return GPR_OK;
+    };
+    int r = GPR_OK;
+    do {
+        // the rule's run: features of every step
+        double t0 = 0;
+        g->pol_feat_log.clear();
+        g->pol_log = true;
+        r = run(0x7fffffff, 0, 1, &t0);
+        g->pol_log = false;
+        if (r) break;
+        const int steps = (int)(g->pol_feat_log.size() / 6);
+        const std::vector<float> feats = g->pol_feat_log;
+        const int take = std::min(steps, cap);
+        for (int j = 0; j < take && !r; ++j) {
+            // evenly spread, first and last step included
+            const int i = take == 1 ? 0 : (int)((int64_t)j * (steps - 1) / (take - 1));
+            for (int a = 0; a < 4 && !r; ++a) {
+                double ms = 0;
+                if ((r = run(i, a, 2, &ms))) break;
+                if (times) times[j * 4 + a] = (float)ms;
             }
-            if (r) break;
             if (features)
-                for (int i = 0; i < 6; ++i) features[cnt * 6 + i] = f[i];
-            ++cnt;
-            // advance with the fastest choice
-            if ((r = gpr_vertex_map(g, GPR_VMAP_COPY, P, Pb, nullptr, 0))) break;
-            if ((r = restore_frontier(g, F, fs))) break;
-            if ((r = step(F, O, bestc, iter, nullptr))) break;
-            U -= (double)F->nacte;
-            std::swap(F, O);
-            ++iter;
+                for (int q = 0; q < 6; ++q) features[j * 6 + q] = feats[(size_t)i * 6 + q];
+            *k = j + 1;
         }
     } while (0);
-    free_copy(fs);
-    free_copy(os);
-    if (P) gpr_prop_destroy(P);
-    if (Pb) gpr_prop_destroy(Pb);
-    if (Pb2) gpr_prop_destroy(Pb2);
-    if (F) gpr_frontier_destroy(F);
-    if (O) gpr_frontier_destroy(O);
-    if (O2) gpr_frontier_destroy(O2);
+    g->pol_force_step = -1;
+    g->pol_force_action = 0;
+    g->pol_log = false;
+    g->pol_feat_log.clear();
     g->pol_params = saved;
-    *k = cnt;
     return r;
 }

@@ -3,15 +3,27 @@
 // last workgroup of a traversal kernel, right after the frontier statistics
 // of that step are complete, so the next step's choice costs no launch.
 //
-// Features (all of the frontier just produced):
-//   f0 log2(1 + nactv) / 32          frontier size
-//   f1 log2(1 + nacte) / 40          frontier out-edge sum
-//   f2 log2(1 + unvisited) / 40      edges of vertices not yet settled
+// Features (all of the frontier just produced; the edge counts relative
+// to m, as the threshold rule sees them, so that the features of a state
+// do not move with the size of the graph):
+//   f0 log2(1 + nactv) / 32                        frontier size
+//   f1 1 + log2((1 + nacte) / (1 + m)) / 32        frontier out-edge sum
+//   f2 1 + log2((1 + unvisited) / (1 + m)) / 32    edges of vertices not yet settled
 //   f3 iteration / 64
-//   f4 log2(1 + m/n) / 8             mean degree
-//   f5 log2(1 + maxdeg/(m/n)) / 24   degree skew
-// Outputs (argmax): 0 push + list output, 1 push + bitmap output,
-//                   2 pull + list output, 3 pull + bitmap output.
+//   f4 log2(1 + m/n) / 8                           mean degree
+//   f5 log2(1 + maxdeg/(m/n)) / 24                 degree skew
+// Outputs, the next step's action (argmax, gated):
+//   0 the threshold rule's own step (the judge: direction by the rule,
+//     the caller's fusion setting, the default output representation)
+//   1 an unfused sparse push (where the rule pulls, only while the
+//     frontier's edges stay within twice the rule's threshold)
+//   2 a push that fuses while its levels stay small (GPR_FUSION_SMALL)
+//   3 a pull that writes a list output, where the rule pulls (elsewhere
+//     the rule's step: a pull costs in proportion to n, which the
+//     features do not carry)
+// Gate: another action replaces the rule's only when its logit exceeds the
+// rule's by at least the margin loaded with the network (calibrated on
+// held-back traces), so an uncertain network takes the rule's step.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,9 +44,11 @@ struct MlpView {
 // Constants of the graph the features need, filled on the host.
 struct PolicyCtx {
     const float *params;      // device, MLP_PARAMS floats (null: no policy)
+    float margin;             // gate: logit lead an action needs over action 0
     float mean_deg_feat;      // f4
     float skew_feat;          // f5
     float iter_feat;          // f3 for the NEXT step
+    float log_m;              // log2(1 + m)
     double unvisited_before;  // unvisited edges before this step
 };
 
@@ -42,88 +56,61 @@ __device__ __forceinline__ void policy_features(const PolicyCtx &c, uint32_t nac
     double unv = c.unvisited_before - (double)nacte;
     if (unv < 0) unv = 0;
     f[0] = log2f(1.0f + (float)nactv) / 32.0f;
-    f[1] = log2f(1.0f + (float)nacte) / 40.0f;
-    f[2] = (float)log2(1.0 + unv) / 40.0f;
+    f[1] = 1.0f + (log2f(1.0f + (float)nacte) - c.log_m) / 32.0f;
+    f[2] = 1.0f + ((float)log2(1.0 + unv) - c.log_m) / 32.0f;
     f[3] = c.iter_feat;
     f[4] = c.mean_deg_feat;
     f[5] = c.skew_feat;
 }
 
-// The whole (last) workgroup evaluates the network: the parameters are
-// staged in LDS with one load round, thread j computes hidden unit j.
-// Every thread of the workgroup must call it; returns the argmax.
-struct PolicyLds {
-    float p[MLP_PARAMS];
-    float h[2 * MLP_H];
-    float z[MLP_OUT];
-};
-__device__ __forceinline__ int policy_forward_block(const float *params, const float f[6], PolicyLds &L) {
-    for (int i = threadIdx.x; i < MLP_PARAMS; i += blockDim.x) L.p[i] = params[i];
-    __syncthreads();
-    const MlpView P(L.p);
-    const int j = threadIdx.x;
-    if (j < MLP_H) {
-        float a = P.b1[j];
-#pragma unroll
-        for (int i = 0; i < MLP_IN; ++i) a += f[i] * P.w1[i * MLP_H + j];
-        L.h[j] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    if (j < MLP_H) {
-        float a = P.b2[j];
-#pragma unroll 8
-        for (int i = 0; i < MLP_H; ++i) a += L.h[i] * P.w2[i * MLP_H + j];
-        L.h[MLP_H + j] = a > 0.f ? a : 0.f;
-    }
-    __syncthreads();
-    if (j < MLP_OUT) {
-        float a = P.b3[j];
-#pragma unroll 8
-        for (int i = 0; i < MLP_H; ++i) a += L.h[MLP_H + i] * P.w3[i * MLP_OUT + j];
-        L.z[j] = a;
-    }
-    __syncthreads();
+// The gated choice from the output logits: the argmax, unless it is not
+// action 0 and leads action 0 by less than the margin.
+__host__ __device__ __forceinline__ int policy_gate(const float z[MLP_OUT], float margin) {
     int best = 0;
     for (int k = 1; k < MLP_OUT; ++k)
-        if (L.z[k] > L.z[best]) best = k;
-    return best;
+        if (z[k] > z[best]) best = k;
+    return best != 0 && z[best] - z[0] < margin ? 0 : best;
 }
 
-// One wave (64 lanes, all active) evaluates the network with lane j holding
-// hidden unit j (activations move between lanes by shuffles); returns the
-// argmax in every lane.
-__device__ __forceinline__ int policy_forward_wave(const float *params, const float f[6]) {
-    const MlpView P(params);
-    const unsigned ln = __lane_id();
-    float h1 = 0.f;
-    if (ln < MLP_H) {
-        float a = P.b1[ln];
+// One wave (64 lanes, all active) evaluates the network: lane j holds
+// hidden unit j (j < 32; the upper half repeats the lower), activations
+// move between lanes by shuffles.  load() issues every weight a lane needs
+// (73 of them) in one batch, so that a caller can issue its own loads
+// (the frontier counts) behind them and wait once for all; run() returns
+// the gated choice in every lane.
+struct MlpWave {
+    float w1[MLP_IN], w2[MLP_H], w3[MLP_H], b1, b2, b3;
+    __device__ __forceinline__ void load(const float *params) {
+        const MlpView P(params);
+        const unsigned ln = __lane_id();
+        const unsigned j = ln & (MLP_H - 1), k = ln & (MLP_OUT - 1);
 #pragma unroll
-        for (int i = 0; i < MLP_IN; ++i) a += f[i] * P.w1[i * MLP_H + ln];
-        h1 = a > 0.f ? a : 0.f;
+        for (int i = 0; i < MLP_IN; ++i) w1[i] = P.w1[i * MLP_H + j];
+#pragma unroll
+        for (int i = 0; i < MLP_H; ++i) w2[i] = P.w2[i * MLP_H + j];
+#pragma unroll
+        for (int i = 0; i < MLP_H; ++i) w3[i] = P.w3[i * MLP_OUT + k];
+        b1 = P.b1[j];
+        b2 = P.b2[j];
+        b3 = P.b3[k];
     }
-    float a2 = ln < MLP_H ? P.b2[ln] : 0.f;
-    for (int i = 0; i < MLP_H; ++i) {
-        const float hi = __shfl(h1, i, 64);
-        if (ln < MLP_H) a2 += hi * P.w2[i * MLP_H + ln];
+    __device__ __forceinline__ int run(const float f[6], float margin) const {
+        float a1 = b1;
+#pragma unroll
+        for (int i = 0; i < MLP_IN; ++i) a1 += f[i] * w1[i];
+        const float h1 = a1 > 0.f ? a1 : 0.f;
+        float a2 = b2;
+#pragma unroll
+        for (int i = 0; i < MLP_H; ++i) a2 += __shfl(h1, i, 64) * w2[i];
+        const float h2 = a2 > 0.f ? a2 : 0.f;
+        float a3 = b3;
+#pragma unroll
+        for (int i = 0; i < MLP_H; ++i) a3 += __shfl(h2, i, 64) * w3[i];
+        float z[MLP_OUT];
+#pragma unroll
+        for (int q = 0; q < MLP_OUT; ++q) z[q] = __shfl(a3, q, 64);
+        return policy_gate(z, margin);
     }
-    const float h2 = a2 > 0.f ? a2 : 0.f;
-    float z = ln < MLP_OUT ? P.b3[ln] : -3.0e38f;
-    for (int i = 0; i < MLP_H; ++i) {
-        const float hi = __shfl(h2, i, 64);
-        if (ln < MLP_OUT) z += hi * P.w3[i * MLP_OUT + ln];
-    }
-    // argmax over lanes 0..3 (ties to the lower index)
-    int best = 0;
-    float bz = __shfl(z, 0, 64);
-    for (int k = 1; k < MLP_OUT; ++k) {
-        const float zk = __shfl(z, k, 64);
-        if (zk > bz) {
-            bz = zk;
-            best = k;
-        }
-    }
-    return best;
-}
+};
 
 } // namespace gpr

@@ -1,6 +1,7 @@
 // runtime.hip -- errors, device selection, allocation, counters, profiling.
 #include "internal.hpp"
 #include "device.hpp"
+#include "grid_sync.hpp"
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -40,6 +41,13 @@ __global__ void k_reset_counters(DevCounters *c) {
         c->edges = 0;
         c->listpos = 0;
     }
+    if (threadIdx.x < CTR_STRIPES) {
+        CtrStripe *st = reinterpret_cast<CtrStripe *>(c + 1) + threadIdx.x;
+        st->nacte = 0;
+        st->edges = 0;
+        st->nactv = 0;
+        st->ticket = 0;
+    }
 }
 
 int reset_counters(gpr_graph *g) {
@@ -48,8 +56,25 @@ int reset_counters(gpr_graph *g) {
     return GPR_OK;
 }
 
+// One wave sums the stripes (lane i reads stripe i) and writes the totals,
+// the list position and the policy decision to the host mirror.
+__global__ void __launch_bounds__(64) k_read_counters(const DevCounters *c, DevCounters *h) {
+    const CtrStripe *st = reinterpret_cast<const CtrStripe *>(c + 1) + threadIdx.x;
+    const unsigned long long ne = wave_sum(st->nacte), ed = wave_sum(st->edges);
+    const unsigned int nv = wave_sum(st->nactv);
+    if (threadIdx.x == 0) {
+        h->nacte = ne;
+        h->edges = ed;
+        h->nactv = nv;
+        h->listpos = c->listpos;
+        h->decision = c->decision;
+    }
+}
+
 int read_counters(gpr_graph *g) {
-    GPR_HIP(hipMemcpyAsync(g->hctr, g->ctr, sizeof(DevCounters), hipMemcpyDeviceToHost, g->stream));
+    static_assert(CTR_STRIPES == 64, "one stripe per lane of the read-back wave");
+    hipLaunchKernelGGL(k_read_counters, dim3(1), dim3(64), 0, g->stream, g->ctr, g->hctr_dev);
+    GPR_HIP(hipGetLastError());
     GPR_HIP(hipStreamSynchronize(g->stream));
     return GPR_OK;
 }
@@ -87,15 +112,16 @@ k_deterministic_sum(const double *__restrict__ parts, uint32_t count, double *sl
         if (threadIdx.x < (unsigned)o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
+    // the slice as an agent-scope store, acknowledged before the ticket; the
+    // last workgroup reads the slices with agent-scope loads (no fence)
     if (threadIdx.x == 0) {
-        slices[blockIdx.x] = red[0];
-        __threadfence();
+        __hip_atomic_store(&slices[blockIdx.x], red[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
         unsigned t = atomicAdd(ticket, 1u);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!last) return;
-    __threadfence();
     if (threadIdx.x == 0) {
         double tot = 0.0;
         for (unsigned i = 0; i < gridDim.x; ++i)
@@ -205,6 +231,7 @@ static double *tunable(const char *name) {
     if (!strcmp(name, "fusion_blocks")) return &t.fusion_blocks;
     if (!strcmp(name, "pr_write_stage_cells")) return &t.pr_write_stage_cells;
     if (!strcmp(name, "fusion_small_limit")) return &t.fusion_small_limit;
+    if (!strcmp(name, "fusion_cooperative")) return &t.fusion_cooperative;
     return nullptr;
 }
 

@@ -1,20 +1,33 @@
-"""Training and evaluation of the learned direction/representation policy.
+"""Training, gate calibration and evaluation of the learned policy.
 
-Pipeline (all on the GPU through the C ABI):
+The policy picks each BFSLVLv / CCv step's action (policy_dev.hpp): 0 the
+threshold rule's own step (the judge), 1 an unfused push (where the rule
+pulls, only up to twice its threshold), 2 a push fused while its levels
+stay small, 3 a pull writing a list where the rule pulls.  Pipeline (all
+on the GPU through the C ABI):
   1. training graphs: RMAT scale 18-22 with a in [0.45, 0.65] (b = c keep
      the 0.19:0.19:0.05 proportions of the remaining mass) generated on the
-     device, and 2-D grids whose edges are rewired with probability p;
-     every graph is relabelled by degree as the benchmarks are;
-  2. traces: gpr_policy_trace times the four choices of every BFS and CC
-     iteration from a snapshot (each plus the better next step);
-  3. training: soft targets softmax(-(t/t_min - 1)/tau) over the four
-     measured times, sample weight = time at stake (mean - min, ms), full-
-     batch Adam on the GPU (gpr_mlp_train);
-  4. evaluation on held-out graphs (RMAT-25 ef16, 4096x4096 grid with 1%
-     random long edges) against the threshold rule, which stays the
-     fallback when no network is loaded.
+     device, and 2-D grids whose edges are rewired with probability p; every
+     graph is relabelled by degree as the benchmarks are.  Two of them are
+     held back as the validation split;
+  2. traces: gpr_policy_trace runs the benchmark under the rule, then again
+     with one step forced to each action, at up to `cap` evenly spread
+     steps; the time from the start of that step to the end of the run
+     says what each action at that step costs;
+  3. targets: the fastest action where it saves at least `gain` of the
+     rule's time to the end and at least `floor_ms`, else the rule's;
+     sample weight = the time at stake (ms between the slowest and the
+     fastest action);
+  4. training: full-batch Adam on the GPU (gpr_mlp_train);
+  5. gate: on the validation split, the margin (logit lead an action needs
+     over the rule's) with the best geometric-mean speedup among those whose
+     worst run stays within 3% of the rule;
+  6. evaluation on held-out graphs (RMAT-25 ef16, a 4096x4096 grid with 1%
+     random long edges) against the rule, outputs compared for equality.
 
-Run as ``python -m graptor_amd.policy train|eval``.
+Run as ``python -m graptor_amd.policy train|eval``; training writes
+graptor_amd/policy_weights.json (parameters, margin, and the record of
+steps 2-6).
 """
 import argparse
 import json
@@ -24,10 +37,11 @@ import time
 
 import numpy as np
 
-from . import DIR_AUTO, DIR_POLICY, Graph, mlp_param_count, mlp_train
+from . import DIR_AUTO, DIR_POLICY, Graph, mlp_forward, mlp_param_count, mlp_train
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 WEIGHTS = os.path.join(HERE, "policy_weights.json")
+ACTIONS = ("rule", "push", "push fused while small", "pull to list")
 
 
 def log(*a):
@@ -67,9 +81,10 @@ def grid_csr(w, h, rewire=0.0, long_frac=0.0, seed=0):
 
 
 def training_graphs(quick=False):
-    """(name, constructor) pairs of the training set."""
+    """(name, constructor) pairs: the training graphs, then the validation
+    split (names ending in /val)."""
     out = []
-    scales = [18, 19, 20] if quick else [18, 19, 20, 21, 22]
+    scales = [18, 20] if quick else [18, 19, 20, 21, 22]
     avals = [0.45, 0.57, 0.65] if quick else [0.45, 0.5, 0.55, 0.6, 0.65]
     for i, s in enumerate(scales):
         for j, a in enumerate(avals):
@@ -77,103 +92,15 @@ def training_graphs(quick=False):
                 continue
             out.append(("rmat%d_a%.2f" % (s, a), lambda s=s, a=a, i=i, j=j: Graph.rmat(s, 16, *rmat_abc(a),
                                                                                        seed=1000 + 31 * i + j)))
-    for side, p in ((512, 0.001), (512, 0.01), (1024, 0.001), (1024, 0.01), (1024, 0.05)):
-        if quick and side == 1024 and p != 0.01:
-            continue
+    grids = ((512, 0.01), (1024, 0.01)) if quick else ((512, 0.001), (512, 0.01), (1024, 0.002), (1024, 0.01),
+                                                       (1024, 0.05), (2048, 0.01), (3072, 0.002), (3072, 0.01))
+    for side, p in grids:
         out.append(("grid%d_p%g" % (side, p), lambda side=side, p=p: Graph.from_csr(*grid_csr(side, side, rewire=p,
                                                                                                 seed=side + int(p * 1e4)))))
+    # validation split: graphs of both kinds the network does not train on
+    out.append(("rmat21_a0.52/val", lambda: Graph.rmat(21, 16, *rmat_abc(0.52), seed=7777)))
+    out.append(("grid1536_p0.005/val", lambda: Graph.from_csr(*grid_csr(1536, 1536, rewire=0.005, seed=1536))))
     return out
-
-
-def heuristic_choice(f, n, m):
-    """The threshold rule's choice for a sample (api.h:150-153 and the
-    engine's default output kinds), recovered from the features."""
-    nactv = np.exp2(f[:, 0] * 32.0) - 1.0
-    nacte = np.exp2(f[:, 1] * 40.0) - 1.0
-    sparse = nactv + nacte <= m / 20.0
-    return np.where(sparse, np.where(nacte > n, 1, 0), 3)
-
-
-def collect(quick=False, sources=3):
-    feats, times, tags, heur = [], [], [], []
-    for name, make in training_graphs(quick):
-        t0 = time.time()
-        g = make()
-        g.reorder()
-        rng = np.random.default_rng(len(name))
-        srcs = [g.highest_degree_vertex()] + list(rng.integers(0, g.n, sources - 1))
-        for s in srcs:
-            f, t = g.policy_trace(0, int(s))
-            feats.append(f)
-            times.append(t)
-            heur.append(heuristic_choice(f, g.n, g.m))
-            tags += ["%s/bfs" % name] * len(f)
-        f, t = g.policy_trace(1)
-        feats.append(f)
-        times.append(t)
-        heur.append(heuristic_choice(f, g.n, g.m))
-        tags += ["%s/cc" % name] * len(f)
-        log("traced %s: n=%d m=%d (%.1fs)" % (name, g.n, g.m, time.time() - t0))
-        g.destroy()
-    return np.concatenate(feats), np.concatenate(times), tags, np.concatenate(heur)
-
-
-def targets_from_times(times, heur=None, tau=0.05, margin=0.03):
-    """Soft targets over the four measured times; where the threshold rule's
-    own choice is within `margin` of the fastest, the target is that choice
-    (the network departs from the rule only for a measured gain)."""
-    t = np.asarray(times, np.float64)
-    tmin = t.min(1, keepdims=True)
-    z = -(t / tmin - 1.0) / tau
-    z -= z.max(1, keepdims=True)
-    p = np.exp(z)
-    p /= p.sum(1, keepdims=True)
-    if heur is not None:
-        th = t[np.arange(len(t)), heur]
-        keep = th <= (1.0 + margin) * tmin[:, 0]
-        p[keep] = 0.0
-        p[keep, heur[keep]] = 1.0
-    w = t.mean(1) - t.min(1) + 1e-3
-    return p.astype(np.float32), w.astype(np.float32)
-
-
-def init_params(seed=0):
-    rng = np.random.default_rng(seed)
-    k = mlp_param_count()
-    p = np.zeros(k, np.float32)
-    o = 0
-    for fan_in, fan_out in ((6, 32), (32, 32), (32, 4)):
-        p[o:o + fan_in * fan_out] = rng.standard_normal(fan_in * fan_out) * np.sqrt(2.0 / fan_in)
-        o += fan_in * fan_out + fan_out
-    return p
-
-
-def train(quick=False, epochs=3000, lr=3e-3, out=WEIGHTS):
-    f, t, tags, heur = collect(quick)
-    T, W = targets_from_times(t, heur)
-    p, loss = mlp_train(init_params(), f, T, W, epochs=epochs, lr=lr)
-    pred = np.argmax(p_logits(p, f), 1)
-    acc = float((pred == np.argmin(t, 1)).mean())
-    # measured time of the chosen actions on the training traces
-    rows = np.arange(len(t))
-    cost = {"learned": float(t[rows, pred].sum()), "threshold": float(t[rows, heur].sum()),
-            "oracle": float(t.min(1).sum())}
-    meta = {"samples": int(len(f)), "loss": float(loss), "train_argmin_accuracy": acc, "trace_cost_ms": cost,
-            "graphs": sorted(set(x.split("/")[0] for x in tags)), "epochs": epochs, "lr": lr}
-    with open(out, "w") as fh:
-        json.dump({"params": [float(x) for x in p], "meta": meta}, fh)
-    log("trained on %d samples: loss %.4f, argmin accuracy %.3f" % (len(f), loss, acc))
-    return p, meta
-
-
-def p_logits(p, f):
-    from . import mlp_forward
-    return mlp_forward(p, f)
-
-
-def load(path=WEIGHTS):
-    with open(path) as fh:
-        return np.array(json.load(fh)["params"], np.float32)
 
 
 def held_out_graphs():
@@ -181,51 +108,188 @@ def held_out_graphs():
             ("grid4096_long1pct", lambda: Graph.from_csr(*grid_csr(4096, 4096, long_frac=0.01, seed=4096)))]
 
 
-def evaluate(params, reps=3, graphs=None):
-    """Median run time of BFS (3 sources) and CC with the threshold rule and
-    with the network; outputs are compared for equality."""
-    rows = []
-    for name, make in graphs or held_out_graphs():
+def sources(g, k=3, seed=7):
+    """The highest-degree vertex and k-1 seeded vertices with edges."""
+    index, _ = g.download()
+    deg = np.diff(index)
+    cand = np.flatnonzero(deg > 0)
+    rng = np.random.default_rng(seed)
+    return [g.highest_degree_vertex()] + [int(v) for v in g.order()[rng.choice(cand, k - 1, replace=False)]]
+
+
+def collect(graphs, cap=32):
+    """Traces of every graph: BFS from three sources and CC."""
+    feats, times, tags = [], [], []
+    for name, make in graphs:
+        t0 = time.time()
         g = make()
         g.reorder()
-        rng = np.random.default_rng(7)
-        srcs = [g.highest_degree_vertex()] + [int(x) for x in rng.integers(0, g.n, 2)]
-        for algo in ["bfs"] * len(srcs) + ["cc"]:
-            s = srcs.pop(0) if algo == "bfs" else None
-            res = {}
-            for mode in ("threshold", "learned"):
-                g.set_policy(params if mode == "learned" else None)
-                d = DIR_POLICY if mode == "learned" else DIR_AUTO
-                ts, out = [], None
-                for _ in range(reps + 1):
-                    o, info = g.bfslvl(s, direction=d) if algo == "bfs" else g.ccv(direction=d)
-                    ts.append(info.ms)
-                    out = o
-                res[mode] = (float(np.median(ts[1:])), out)
-            same = bool(np.array_equal(res["threshold"][1], res["learned"][1]))
-            rows.append({"graph": name, "algo": algo, "source": s, "threshold_ms": res["threshold"][0],
-                         "learned_ms": res["learned"][0], "speedup": res["threshold"][0] / res["learned"][0],
-                         "bit_exact": same})
-            log("%s %s: threshold %.3f ms, learned %.3f ms, x%.3f, equal=%s" % (
-                name, algo, res["threshold"][0], res["learned"][0], rows[-1]["speedup"], same))
-        g.set_policy(None)
+        for algo, s in [(0, s) for s in sources(g)] + [(1, 0)]:
+            f, t = g.policy_trace(algo, s, cap=cap)
+            feats.append(f)
+            times.append(t)
+            tags += ["%s/%s" % (name, "bfs" if algo == 0 else "cc")] * len(f)
+        log("traced %s: n=%d m=%d (%.1fs)" % (name, g.n, g.m, time.time() - t0))
+        g.destroy()
+    return np.concatenate(feats), np.concatenate(times), tags
+
+
+def targets_from_times(times, gain=0.02, floor_ms=0.015, smooth=0.05):
+    """Targets: the fastest action where it saves at least `gain` of the
+    rule's time to the end of the run and at least `floor_ms`, else the
+    rule's (action 0), smoothed; weights: the time at stake (ms between the
+    slowest and the fastest action)."""
+    t = np.asarray(times, np.float64)
+    best = np.argmin(t, 1)
+    tb = t[np.arange(len(t)), best]
+    take = (tb <= (1.0 - gain) * t[:, 0]) & (t[:, 0] - tb >= floor_ms)
+    lab = np.where(take, best, 0)
+    T = np.full(t.shape, smooth / (t.shape[1] - 1))
+    T[np.arange(len(t)), lab] = 1.0 - smooth
+    W = t.max(1) - t.min(1) + 1e-2
+    return T.astype(np.float32), W.astype(np.float32), lab
+
+
+def init_params(seed=0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(mlp_param_count(), np.float32)
+    o = 0
+    for fan_in, fan_out in ((6, 32), (32, 32), (32, 4)):
+        p[o:o + fan_in * fan_out] = rng.standard_normal(fan_in * fan_out) * np.sqrt(2.0 / fan_in)
+        o += fan_in * fan_out + fan_out
+    return p
+
+
+def gated(z, margin):
+    """The engine's gate (policy_dev.hpp policy_gate) on logits z (k, 4)."""
+    best = np.argmax(z, 1)
+    lead = z[np.arange(len(z)), best] - z[:, 0]
+    return np.where((best != 0) & (lead < margin), 0, best)
+
+
+def run_pair(g, algo, s, params, reps):
+    """Median run time under the rule and under the network, the runs of
+    the two alternating (one warm run of each first, so that a drift of the
+    clock over the runs affects both alike), and whether the outputs are
+    equal."""
+    ts = {"rule": [], "learned": []}
+    out = {}
+    for i in range(reps + 1):
+        for mode in ("rule", "learned"):
+            g.set_policy(params if mode == "learned" else None)
+            d = DIR_POLICY if mode == "learned" else DIR_AUTO
+            o, info = g.bfslvl(s, direction=d) if algo == "bfs" else g.ccv(direction=d)
+            if i:
+                ts[mode].append(info.ms)
+            out[mode] = o
+    g.set_policy(None)
+    return (float(np.median(ts["rule"])), float(np.median(ts["learned"])),
+            bool(np.array_equal(out["rule"], out["learned"])))
+
+
+def steps_text(trace):
+    """A run's steps: P push / L pull, f fused (with its level count), the
+    input frontier size, > a list output."""
+    return " ".join("%s%s%d%s" % ("P" if t["direction"] == 1 else "L",
+                                  "f%d:" % t["levels"] if t["fused"] else ":", t["nactv_in"],
+                                  ">" if t["out_kind"] == 2 else "") for t in trace)
+
+
+def evaluate(params, graphs, reps=5):
+    """BFS from three sources and CC on each graph: median times with the
+    rule and with the network (one warm run first)."""
+    rows = []
+    for name, make in graphs:
+        g = make()
+        g.reorder()
+        runs = [("bfs", s) for s in sources(g)] + [("cc", None)]
+        for algo, s in runs:
+            tr, tl, same = run_pair(g, algo, s, params, reps)
+            g.set_policy(params)
+            _ = g.bfslvl(s, direction=DIR_POLICY, trace=True) if algo == "bfs" else g.ccv(direction=DIR_POLICY,
+                                                                                          trace=True)
+            steps = steps_text(g.trace())
+            g.set_policy(None)
+            rows.append({"graph": name, "algo": algo, "source": s, "rule_ms": tr, "learned_ms": tl,
+                         "speedup": tr / tl, "bit_exact": same, "steps": steps})
+            log("%s %s %s: rule %.3f ms, learned %.3f ms, x%.3f, equal=%s | %s" % (name, algo, s, tr, tl, tr / tl,
+                                                                                 same, steps))
         g.destroy()
     sp = np.array([r["speedup"] for r in rows])
     return {"rows": rows, "geomean_speedup": float(np.exp(np.log(sp).mean())), "worst_speedup": float(sp.min()),
             "all_bit_exact": all(r["bit_exact"] for r in rows)}
 
 
+def calibrate(p, graphs, margins=(0.0, 0.5, 1.0, 2.0, 3.0, 5.0, 8.0, 1e9), reps=5, worst_ok=0.97):
+    """The gate margin: best geomean on the validation graphs among the
+    margins whose worst run is within 3% of the rule (the largest margin
+    tried, which keeps almost every rule step, if none is)."""
+    table = []
+    for m in margins:
+        ev = evaluate(np.concatenate([p, [m]]).astype(np.float32), graphs, reps=reps)
+        table.append({"margin": m, "geomean": ev["geomean_speedup"], "worst": ev["worst_speedup"],
+                      "bit_exact": ev["all_bit_exact"]})
+        log("margin %.2f: geomean %.3f worst %.3f" % (m, ev["geomean_speedup"], ev["worst_speedup"]))
+    ok = [r for r in table if r["worst"] >= worst_ok] or [table[-1]]
+    best = max(ok, key=lambda r: r["geomean"])
+    return best["margin"], table
+
+
+def train(quick=False, epochs=4000, lr=3e-3, cap=32, out=WEIGHTS, seed=0):
+    allg = training_graphs(quick)
+    tr = [x for x in allg if not x[0].endswith("/val")]
+    va = [x for x in allg if x[0].endswith("/val")]
+    f, t, tags = collect(tr, cap)
+    T, W, lab = targets_from_times(t)
+    p, loss = mlp_train(init_params(seed), f, T, W, epochs=epochs, lr=lr)
+    z = mlp_forward(p, f)
+    pred = np.argmax(z, 1)
+    rows = np.arange(len(t))
+    cost = {"learned": float(t[rows, pred].sum()), "rule": float(t[rows, 0].sum()), "best": float(t.min(1).sum())}
+    for tag in sorted(set(tags)):
+        sel = np.array([x == tag for x in tags])
+        log("  labels %-24s %s" % (tag, " ".join(str(int((lab[sel] == a).sum())) for a in range(4))))
+    log("trained on %d samples (%s): loss %.4f, label accuracy %.3f" % (
+        len(f), ", ".join("%s %d" % (ACTIONS[a], int((lab == a).sum())) for a in range(4)), loss,
+        float((pred == lab).mean())))
+    margin, table = calibrate(p, va)
+    meta = {"samples": int(len(f)), "labels": [int((lab == a).sum()) for a in range(4)], "loss": float(loss),
+            "label_accuracy": float((pred == lab).mean()), "trace_cost_ms": cost, "epochs": epochs, "lr": lr,
+            "cap": cap, "graphs": sorted(set(x.split("/")[0] for x in tags)), "validation": [x[0] for x in va],
+            "calibration": table, "margin": margin}
+    with open(out, "w") as fh:
+        json.dump({"params": [float(x) for x in p], "margin": margin, "meta": meta}, fh)
+    log("margin %.2f; written %s" % (margin, out))
+    return p, margin, meta
+
+
+def load(path=WEIGHTS):
+    """Parameters followed by the gate margin, as gpr_policy_set takes them."""
+    with open(path) as fh:
+        d = json.load(fh)
+    return np.array(list(d["params"]) + [d.get("margin", 0.0)], np.float32)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--quick", action="store_true")
-    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--cap", type=int, default=32)
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
     if args.cmd == "train":
-        train(args.quick, args.epochs, out=args.out)
+        train(args.quick, args.epochs, cap=args.cap, out=args.out)
+        ev = evaluate(load(args.out), held_out_graphs())
+        with open(args.out) as fh:
+            d = json.load(fh)
+        d["meta"]["held_out"] = ev
+        with open(args.out, "w") as fh:
+            json.dump(d, fh)
+        print(json.dumps({"geomean_speedup": ev["geomean_speedup"], "worst_speedup": ev["worst_speedup"],
+                          "all_bit_exact": ev["all_bit_exact"]}))
     else:
-        print(json.dumps(evaluate(load(args.out))))
+        print(json.dumps(evaluate(load(args.out), held_out_graphs())))
 
 
 if __name__ == "__main__":

@@ -199,7 +199,13 @@ typedef struct {
      * launch, until none is left; its output frontier is then empty.  For
      * GPR_RELAX_MIN_PLUS1 / GPR_RELAX_MIN; the dst filter is implied by the
      * relax.  GPR_FUSION_OFF, _ON (fusion_select(true)), _DEFAULT
-     * (default_fusion_select: when the frontier has >= 4096 edges). */
+     * (default_fusion_select: when the frontier has >= 4096 edges; on this
+     * engine also only while the frontier and each level it reaches stay
+     * within fusion_small_limit, as under GPR_FUSION_SMALL below: the fused
+     * kernel runs one workgroup per compute unit, and a larger level costs
+     * it more than an unfused step does, so the first such level is handed
+     * back as the step's output frontier).  Results do not depend on the
+     * fusion setting. */
     int fusion;
 } gpr_emap_desc;
 
@@ -222,6 +228,9 @@ typedef struct {
                                relaxation of a fused step, repeats included) */
     double ms;            /* device time of the step */
     int fused;            /* 1: a fused push step (api::fusion) */
+    int levels;           /* levels the step advanced: 1 for an unfused
+                             step; a fused step's handed-back frontier lies
+                             this many levels past its input */
 } gpr_iter_stats;
 
 /* One edgemap step over frontier `in`, recording into `out` (which may be
@@ -395,18 +404,35 @@ int gpr_batch_cc_noalloc(const uint64_t *d_index, const uint32_t *d_nbrs,
 
 /* ---- learned direction / representation policy (GPR_DIR_POLICY) --------
  * No counterpart in the reference, whose rule is the fixed threshold of
- * api.h:150-153 (kept as the fallback when no network is loaded).  A
- * 6-32-32-4 FP32 MLP picks {push, pull} x {list, bitmap} output each
- * step; its forward pass runs in the last workgroup of the step's kernel.
+ * api.h:150-153 (the fallback when no network is loaded, and the judge).
+ * A 6-32-32-4 FP32 MLP picks each step's action: 0 = the threshold rule's
+ * own step (direction by the rule, the caller's fusion setting and output
+ * representation), 1 = an unfused sparse push (where the rule pulls, only
+ * while vertices + out-edges of the frontier stay within twice the rule's
+ * threshold, else the rule's step), 2 = a push fused while its levels stay
+ * small (GPR_FUSION_SMALL), 3 = a pull with a list output where the rule
+ * pulls (the rule's step where it pushes).  Features: log2 of the frontier
+ * size, of its out-edges and of the unvisited edges (both relative to m),
+ * the iteration, log mean degree and log degree skew.  Its forward pass
+ * runs in the last workgroup of the step's kernel (on the host after a
+ * fused step).  An action other than 0 is taken only when its logit leads
+ * action 0's by at least the margin.
  * Parameter layout: W1[6][32] b1[32] W2[32][32] b2[32] W3[32][4] b3[4]. */
 int gpr_mlp_param_count(void);
-/* Load (params != NULL) or drop (NULL) the network of this graph handle. */
+/* Load (params != NULL) or drop (NULL) the network of this graph handle:
+ * count = gpr_mlp_param_count() (margin 0), or one more value, the
+ * margin. */
 int gpr_policy_set(gpr_graph *g, const float *params, int count);
 /* Start of a run: iteration 0, all edges unvisited, no pending choice. */
 int gpr_policy_reset(gpr_graph *g);
-/* Training traces: algo 0 = BFSLVLv from start, 1 = CCv.  For each of at
- * most cap iterations: the 6 features and the measured ms of the 4 choices
- * (each plus the better next step).  *k receives the iteration count. */
+/* Training traces: algo 0 = BFSLVLv from start, 1 = CCv (min-label
+ * propagation from all vertices).  The run is sampled at up to cap
+ * iterations (evenly spread); for each sample: the 6 features and, for each
+ * of the 4 actions, the measured ms (host clock, best of two runs) from
+ * the start of that step to the end of the run when the step takes the
+ * action and every later step the threshold rule's
+ * (BFSLVLv: no fusion; CCv: default_fusion_select, frac_threshold(5)).
+ * *k receives the sample count. */
 int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
                      float *features, float *times, int *k);
 /* The network on the GPU: logits of count samples; gradient of the
@@ -438,9 +464,9 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
  *                       the step fails with GPR_E_HIP (default 2^22)
  *   fusion_blocks       fused push: blocks of 1024 threads in its
  *                       cooperative grid (0: one per compute unit)
- *   fusion_small_limit  GPR_FUSION_SMALL: the largest frontier (vertices +
- *                       out-edges) that fuses, and the level size that ends
- *                       a fused step (default 2^17)
+ *   fusion_small_limit  GPR_FUSION_SMALL and _DEFAULT: the largest frontier
+ *                       (vertices + out-edges) that fuses, and the level
+ *                       size that ends a fused step (default 2^17)
  *   pr_write_stage_cells  PageRank binned edgemap: a source tile's run
  *                       offsets are staged in LDS when it has at most this
  *                       many non-empty (tile, bin) cells (capped at 4096,

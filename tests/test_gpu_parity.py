@@ -320,6 +320,33 @@ def test_vertex_filter_and_maps():
         p.destroy()
 
 
+@pytest.mark.parametrize("k", [0, 1, 63, 2048, 2049, 100000])
+def test_vertex_filter_list_ordered(k):
+    """vertex_filter over a list (vertexmap.h vertex_filter): the kept
+    entries with their count and out-degree sum, for lists within one
+    workgroup's tile, at its edge and across many."""
+    index, nbrs = O.er_graph(300000, 4, seed=5)
+    n = len(index) - 1
+    rng = np.random.default_rng(k)
+    vs = rng.choice(n, size=k, replace=False).astype(np.uint32)
+    vals = rng.integers(0, 3, size=n, dtype=np.uint32)
+    deg = np.diff(index)
+    with G.Graph.from_csr(index, nbrs) as g:
+        p = g.prop(G.U32)
+        p.upload(vals)
+        f = g.frontier().set_list(vs)
+        for neg in (False, True):
+            out = f.filter(p, 1, neg=neg)
+            keep = vs[(vals[vs] == 1) != neg]
+            nv, ne, kind = out.stats()
+            assert nv == len(keep) and ne == int(deg[keep].sum())
+            assert kind == (G.FRONTIER_LIST if len(keep) else G.FRONTIER_EMPTY)
+            assert np.array_equal(out.download(), np.sort(keep))   # download returns ascending IDs
+            out.destroy()
+        f.destroy()
+        p.destroy()
+
+
 # ---- full-size properties (configs: RMAT-20 BFS; RMAT-24 CC and PR) -----------
 @pytest.mark.parametrize("scale", [20])
 def test_rmat20_bfs_full_size(scale):

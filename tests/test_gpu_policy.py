@@ -5,8 +5,10 @@
     finite difference of the loss confirms that restatement;
   * Adam training on the GPU lowers the loss;
   * BFS and CC stay bit-exact whatever a (random) network chooses, with the
-    choice made by the fused in-kernel inference;
-  * traces record four timed choices per iteration.
+    choice made by the fused in-kernel inference; forced networks are
+    obeyed, and the gate keeps the rule's step below the margin;
+  * traces record the time to finish the run under each of the four
+    actions.
 """
 import numpy as np
 import pytest
@@ -133,30 +135,93 @@ def test_policy_keeps_results_bit_exact(seed):
             assert np.array_equal(lv2, lv)
 
 
+def pushes_where_allowed(tr, rule, m):
+    """Action 1's steps: an unfused push where the rule pushes or the
+    frontier's vertices + out-edges are at most twice the rule's threshold
+    (default_threshold: m / 20), the rule's pull elsewhere."""
+    assert len(tr) == len(rule)
+    for t, r in zip(tr, rule):
+        if r[0] == G.DIR_PUSH or t["nactv_in"] + t["nacte_in"] <= 2 * (m // 20):
+            assert t["direction"] == G.DIR_PUSH and not t["fused"], (t, r)
+        else:
+            assert t["direction"] == G.DIR_PULL, (t, r)
+
+
 def test_policy_choice_follows_network():
-    """A network whose output bias forces one choice is obeyed each step."""
+    """A network whose output bias forces one action is obeyed each step:
+    0 the threshold rule's own step, 1 unfused pushes (within twice the
+    rule's threshold), 2 pushes fused while their levels stay small, 3
+    pulls writing lists."""
     k = G.mlp_param_count()
-    for forced, (d, kind) in enumerate([(G.DIR_PUSH, G.FRONTIER_LIST), (G.DIR_PUSH, G.FRONTIER_BITMAP),
-                                        (G.DIR_PULL, G.FRONTIER_LIST), (G.DIR_PULL, G.FRONTIER_BITMAP)]):
-        p = np.zeros(k, np.float32)
-        p[k - OUT + forced] = 10.0
-        index, nbrs = O.rmat_graph(12)
-        with G.Graph.from_csr(index, nbrs) as g:
+    index, nbrs = O.rmat_graph(12)
+    ref = O.bfs_levels(index, nbrs, 0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        lv, _ = g.bfslvl(0, trace=True)
+        rule = [(t["direction"], t["out_kind"]) for t in g.trace()]
+        for forced in range(4):
+            p = np.zeros(k, np.float32)
+            p[k - OUT + forced] = 10.0
             g.set_policy(p)
             lv, _ = g.bfslvl(0, direction=G.DIR_POLICY, trace=True)
             tr = g.trace()
-            assert all(t["direction"] == d and t["out_kind"] in (kind, G.FRONTIER_EMPTY) or t["nactv_out"] == 0
-                       for t in tr)
-            assert np.array_equal(lv, O.bfs_levels(index, nbrs, 0))
+            assert np.array_equal(lv, ref), forced
+            if forced == 0:
+                assert [(t["direction"], t["out_kind"]) for t in tr] == rule
+            elif forced == 1:
+                pushes_where_allowed(tr, rule, len(nbrs))
+            elif forced == 2:
+                assert all(t["direction"] == G.DIR_PUSH for t in tr) and tr[0]["fused"]
+            else:
+                # a list-writing pull where the rule pulls, the rule's step
+                # where it pushes
+                assert [t["direction"] for t in tr] == [d for d, _ in rule]
+                assert all(t["out_kind"] in (G.FRONTIER_LIST, G.FRONTIER_EMPTY) for t in tr
+                           if t["direction"] == G.DIR_PULL)
+        g.set_policy(None)
 
 
-def test_trace_records_four_choices():
+def test_policy_gate_margin():
+    """An action other than the rule's is taken only with a logit lead of at
+    least the margin loaded after the parameters."""
+    k = G.mlp_param_count()
     index, nbrs = O.rmat_graph(12)
     with G.Graph.from_csr(index, nbrs) as g:
-        f, t = g.policy_trace(0, O.highest_degree_vertex(index))
-        lv, info = g.bfslvl(O.highest_degree_vertex(index))
+        lv, _ = g.bfslvl(0, trace=True)
+        rule = [(t["direction"], t["out_kind"], t["fused"]) for t in g.trace()]
+        p = np.zeros(k, np.float32)
+        p[k - OUT + 1] = 1.0            # an unfused push leads the rule's action by 1
+        for margin, follows in ((0.5, True), (2.0, False)):
+            g.set_policy(np.concatenate([p, [margin]]).astype(np.float32))
+            lv2, _ = g.bfslvl(0, direction=G.DIR_POLICY, trace=True)
+            tr = g.trace()
+            assert np.array_equal(lv2, lv)
+            if follows:
+                pushes_where_allowed(tr, rule, len(nbrs))
+            else:
+                assert [(t["direction"], t["out_kind"], t["fused"]) for t in tr] == rule
+        with pytest.raises(G.GprError):
+            g.set_policy(np.concatenate([p, [-1.0]]).astype(np.float32))
+        g.set_policy(None)
+
+
+def test_trace_records_four_actions():
+    """Each sample times runs with that step forced to each action, from the
+    start of that step to the end of the run; action 0 is the rule's own
+    step, so its times are what the rule takes from there on: the first
+    sample's is a whole run of the rule, and later ones are shorter."""
+    index, nbrs = O.rmat_graph(12)
+    with G.Graph.from_csr(index, nbrs) as g:
+        s = O.highest_degree_vertex(index)
+        f, t = g.policy_trace(0, s)
+        lv, info = g.bfslvl(s)
         assert f.shape[0] == t.shape[0] >= 3 and f.shape[1] == 6 and t.shape[1] == 4
         assert (t > 0).all() and np.isfinite(f).all()
-        assert abs(f.shape[0] - info.iters) <= 1
+        assert f.shape[0] <= info.iters + 1
+        # sample 0 is step 0: a whole run of the rule, close to the
+        # benchmark's own run time; the last sample starts at the last step
+        assert t[0, 0] < 3 * info.ms and info.ms < 3 * t[0, 0]
+        assert t[-1, 0] < t[0, 0]
         f2, t2 = g.policy_trace(1)
         assert len(f2) >= 2
+        f3, t3 = g.policy_trace(0, s, cap=2)
+        assert len(f3) <= 2

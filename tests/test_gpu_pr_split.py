@@ -5,9 +5,10 @@ the edges from the other ("cold") sources through a binned layout with
 fixed-point accumulation (split point 0: every edge binned).  The split is
 set here per test (tunable pr_split_bytes = 4 bytes per pulled source;
 1e12 = no split) so that test-sized graphs take the binned path, with tiles
-capped by source count (32768) and by edges (262144), several destination
-bins (16384 each), long rows (> 2048 in-edges), directed graphs, and (on
-RMAT-20) hub bins of more than 2^20 entries that are summed in chunks.
+capped by source count (32768; the edge cap of 2^24 per tile is above every
+graph here), several destination bins (16384 each), long rows (> 2048
+in-edges), directed graphs, and (on RMAT-20) hub bins of more than 2^20
+entries that are summed in chunks.
 
 Bars: the same as the plain pull (relative L1 vs FP64: FP32 1e-6, 16-bit
 formats 1e-3 on RMAT and 2e-3 on near-regular graphs, truncating bf16
