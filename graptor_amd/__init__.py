@@ -108,6 +108,7 @@ SIGNATURES = {
     "gpr_edgemap": (C.c_int, [_P, C.POINTER(EmapDesc), _P, _P, C.POINTER(IterStats)]),
     "gpr_vertex_map": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_uint64]),
     "gpr_vertex_filter": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_int, _P]),
+    "gpr_pr_iterate_fused": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, _P]),
     "gpr_pr_contrib": (C.c_int, [_P, _P, _P, C.c_int, C.c_float]),
     "gpr_pr_finish": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_float, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)]),

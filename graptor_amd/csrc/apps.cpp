@@ -354,13 +354,38 @@ struct PRv {
         check(gpr_pr_contrib(GA.get(), x_mem.get(), contrib.get(), fmt, dampen), "contributions");
         sdelta = 2.0;
         const bool need_delta = tol > 0.0;
+        // without a convergence test between iterations, each iteration's
+        // tail (but the last) is deferred into the next edgemap's write
+        // stage (gpr_pr_iterate_fused), where the layout allows it
+        bool fuse_tail = !need_delta && GA.numEdges() > 0;
+        bool pending = false;
         while (iter < o.max_iter && sdelta > tol) {
-            auto lazy = A::edgemap(GA, A::relax(A::pr_sum(y_mem, contrib, fmt, dampen)));
-            lazy.materialize();
+            int rc = GPR_E_INVAL;
+            if (pending) {
+                rc = gpr_pr_iterate_fused(GA.get(), x_mem.get(), y_mem.get(), fmt, dampen, nullptr);
+                if (rc == GPR_E_INVAL) {
+                    // not fusable here: the deferred tail, then the plain edgemap
+                    fuse_tail = false;
+                    check(gpr_pr_finish(GA.get(), x_mem.get(), y_mem.get(), contrib.get(), fmt, dampen, nullptr,
+                                        nullptr),
+                          "pr tail");
+                } else {
+                    check(rc, "pr fused iteration");
+                }
+                pending = false;
+            }
+            if (rc != GPR_OK) {
+                auto lazy = A::edgemap(GA, A::relax(A::pr_sum(y_mem, contrib, fmt, dampen)));
+                lazy.materialize();
+            }
             // the vertex_scan / vertex_map tail of PRv.C:214-269, fused
-            check(gpr_pr_finish(GA.get(), x_mem.get(), y_mem.get(), contrib.get(), fmt, dampen,
-                                need_delta ? &sdelta : nullptr, nullptr),
-                  "pr tail");
+            // (deferred into the next iteration when that one is fused)
+            if (fuse_tail && iter + 1 < o.max_iter)
+                pending = true;
+            else
+                check(gpr_pr_finish(GA.get(), x_mem.get(), y_mem.get(), contrib.get(), fmt, dampen,
+                                    need_delta ? &sdelta : nullptr, nullptr),
+                      "pr tail");
             gpr_iter_stats st = {};
             st.nactv_in = GA.numVertices();
             st.nacte_in = GA.numEdges();

@@ -162,6 +162,10 @@ struct ColdLayout {
     uint32_t *groups = nullptr;     // per group of <= COLD_GROUP slots of one bin: first slot, slots, group slot, 0
     long long *partials2 = nullptr; // COLD_BIN fixed-point sums per group
     double *mass = nullptr;         // per write chunk: sum over its tile's edges of |contribution| (first chunk)
+    // sources in no tile's span (no out-edges): ngaps ranges, gaps[0..ngaps)
+    // their first sources, gaps[ngaps..2*ngaps] the prefix of their lengths
+    uint32_t ngaps = 0;
+    uint32_t *gaps = nullptr;
 };
 
 struct Profile {
@@ -296,7 +300,10 @@ int fused_push(gpr_graph *g, int relax, uint32_t *prop, gpr_frontier *in, gpr_fr
 int cold_prepare(gpr_graph *g, uint32_t from);   // (re)build for this split point
 void cold_free(gpr_graph *g);
 int scan_frontier_degrees(gpr_graph *g, const uint32_t *list, uint32_t k);
-int cold_launch_write(gpr_graph *g, int fmt, const void *contrib);
+// x == nullptr: stage the stored contributions; else compute them from y
+// with the previous iteration's tail fused in (x' written, cold_tail_fusable)
+int cold_launch_write(gpr_graph *g, int fmt, const void *contrib, const float *y, float *x, float damping);
+bool cold_tail_fusable(const gpr_graph *g);
 // y (+)= the binned sums; *nparts receives the number of y partial slots
 int cold_launch_accum(gpr_graph *g, int fmt, float *y, int add, double *part, float *part_max, uint32_t *nparts);
 

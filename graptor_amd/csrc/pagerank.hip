@@ -378,7 +378,10 @@ static uint32_t split_point(gpr_graph *g) {
     return h >= (double)g->n ? g->n : (uint32_t)h;
 }
 
-int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st) {
+// x_tail != nullptr: the previous iteration's tail is fused into the write
+// stage (the caller has checked cold_tail_fusable after cold_prepare)
+static int pr_edgemap_impl(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st,
+                           float *x_tail) {
     GPR_REQUIRE(!in || in->kind == GPR_FRONTIER_ALL, "PR edgemap runs over the all-true frontier only");
     GPR_REQUIRE(desc->prop && desc->prop->dtype == GPR_F32, "PR edgemap: accumulator must be an f32 property");
     const int fmt = desc->contrib_fmt;
@@ -401,7 +404,7 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
     GPR_HIP(hipEventRecord(g->ev0, g->stream));
     if (split) {
         ProfScope ps(g, "pr_cold_write");
-        GPR_TRY(cold_launch_write(g, fmt, cb));
+        GPR_TRY(cold_launch_write(g, fmt, cb, y, x_tail, damping));
         hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, g->cold->mass,
                            (const float *)nullptr, (const double *)nullptr, g->cold->nwchunks, g->dsum_slices,
                            &g->ctr->ticket2, g->prs, 3, g->n, damping, 0);
@@ -458,9 +461,34 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
     return GPR_OK;
 }
 
+int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_iter_stats *st) {
+    return pr_edgemap_impl(g, desc, in, st, nullptr);
+}
+
 } // namespace gpr
 
 using namespace gpr;
+
+extern "C" int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt, float damping,
+                                    gpr_iter_stats *st) {
+    if (!g || !x || !y || x->dtype != GPR_F32 || y->dtype != GPR_F32 || x->g != g || y->g != g) return GPR_E_INVAL;
+    GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_SCFP16, "pr_iterate_fused: unknown format %d", fmt);
+    GPR_REQUIRE(split_point(g) == 0, "pr_iterate_fused: needs every edge binned (pr_split_bytes 0)");
+    GPR_TRY(cold_prepare(g, 0));
+    GPR_REQUIRE(cold_tail_fusable(g), "pr_iterate_fused: the graph has no binned edges");
+    gpr_emap_desc d = {};
+    d.relax = GPR_RELAX_PR_SUM;
+    d.prop = y;
+    d.contrib_fmt = fmt;
+    d.damping = damping;
+    // the contribution property is never read: the write stage computes
+    // the contributions; a stand-in of the right element size passes the
+    // checks
+    gpr_prop stand_in = *y;
+    stand_in.elem = (uint32_t)fmt_bytes(fmt);
+    d.contrib = &stand_in;
+    return pr_edgemap_impl(g, &d, nullptr, st, (float *)x->d);
+}
 
 extern "C" int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib, int fmt, float damping) {
     if (!g || !x || !contrib || x->dtype != GPR_F32) return GPR_E_INVAL;

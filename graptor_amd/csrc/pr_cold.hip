@@ -210,6 +210,7 @@ void cold_free(gpr_graph *g) {
     dev_free(L->partials);
     dev_free(L->groups);
     dev_free(L->partials2);
+    dev_free(L->gaps);
     delete L;
     g->cold = nullptr;
 }
@@ -285,6 +286,18 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
     if (ncold == 0 || ncold >= 0xffffffffull) return GPR_OK;
     std::vector<uint32_t> lo;
     make_tiles(idx, from, n, lo);
+    // the sources outside every tile's span [lo[t], lo[t] + w_t): their
+    // ranks are updated by a kernel of their own when the PageRank tail is
+    // fused into the write stage
+    std::vector<uint32_t> gap_lo, gap_len;
+    {
+        uint32_t at = from;
+        for (size_t t = 0; t + 1 < lo.size(); ++t) {
+            if (lo[t] > at) { gap_lo.push_back(at); gap_len.push_back(lo[t] - at); }
+            at = lo[t] + std::min<uint32_t>(lo[t + 1] - lo[t], COLD_TILE_V);
+        }
+        if (n > at) { gap_lo.push_back(at); gap_len.push_back(n - at); }
+    }
     idx.clear();
     idx.shrink_to_fit();
     const uint32_t ntiles = (uint32_t)lo.size() - 1;
@@ -514,6 +527,22 @@ int cold_prepare(gpr_graph *g, uint32_t from) {
         cold_free(g);
         return r;
     }
+    L->ngaps = (uint32_t)gap_lo.size();
+    if (L->ngaps) {
+        std::vector<uint32_t> h(gap_lo);
+        h.push_back(0);
+        for (size_t i = 0; i < gap_len.size(); ++i) h.push_back(h.back() + gap_len[i]);
+        if ((r = dev_alloc((void **)&L->gaps, sizeof(uint32_t) * h.size(), "cold gaps")) == GPR_OK &&
+            hipMemcpyAsync(L->gaps, h.data(), sizeof(uint32_t) * h.size(), hipMemcpyHostToDevice, g->stream) !=
+                hipSuccess)
+            r = GPR_E_HIP;
+        if (r == GPR_OK && hipStreamSynchronize(g->stream) != hipSuccess) r = GPR_E_HIP;
+        if (r != GPR_OK) {
+            if (r == GPR_E_HIP) set_error("PageRank cold-edge layout build failed");
+            cold_free(g);
+            return r;
+        }
+    }
     L->ntiles = ntiles;
     L->nbins = nbins;
     L->ncold = ncold;
@@ -578,28 +607,59 @@ __device__ __forceinline__ void cold_copy(uint32_t p0, uint32_t p1, const uint16
     }
 }
 
+// The previous iteration's vertex-map tail, fused into the staging (the
+// update of gpr_pr_finish without its residual): x' = (y + w) / s and the
+// contribution damping * x' / deg, from the scalars of the last reduction.
+struct PrTail {
+    const float *y;
+    float *x;
+    float damping;
+};
+
 // Stage a tile's w contributions in LDS, CW_STAGE loads per thread at a
-// time (clamped, so that all are in flight together); with MASS, also this
-// thread's share of sum |contribution| x out-degree.
-template <int FMT, bool MASS>
+// time (clamped, so that all are in flight together); with MASS (the
+// tile's first chunk), also this thread's share of sum |contribution| x
+// out-degree.  With UPD the contributions are computed from y (PrTail)
+// exactly as the tail computes them, and the first chunk writes x'.
+template <int FMT, bool MASS, bool UPD>
 __device__ __forceinline__ double cold_stage(const typename Codec<FMT>::store_t *__restrict__ c,
                                              const uint32_t *__restrict__ deg, uint32_t w,
-                                             typename Codec<FMT>::store_t *win) {
+                                             typename Codec<FMT>::store_t *win, const PrTail &tl,
+                                             const PrScalars *__restrict__ prs) {
     using S = typename Codec<FMT>::store_t;
     double ms = 0.0;
+    double pw = 0.0, pinv = 0.0;
+    float sc = 1.0f;
+    if constexpr (UPD) {
+        pw = prs->w;
+        pinv = prs->inv_s;
+        sc = FMT == GPR_CONTRIB_F16 ? prs->enc_scale : 1.0f;
+    }
     for (uint32_t i0 = 0; i0 < w; i0 += CW_THREADS * CW_STAGE) {
         S v[CW_STAGE];
         uint32_t dg[CW_STAGE];
+        float yv[CW_STAGE];
 #pragma unroll
         for (int k = 0; k < CW_STAGE; ++k) {
             const uint32_t i = i0 + k * CW_THREADS + threadIdx.x, ic = i < w ? i : 0u;
-            v[k] = c[ic];
-            if constexpr (MASS) dg[k] = deg[ic];
+            if constexpr (UPD) {
+                yv[k] = tl.y[ic];
+                dg[k] = deg[ic];
+            } else {
+                v[k] = c[ic];
+                if constexpr (MASS) dg[k] = deg[ic];
+            }
         }
 #pragma unroll
         for (int k = 0; k < CW_STAGE; ++k) {
             const uint32_t i = i0 + k * CW_THREADS + threadIdx.x;
             if (i < w) {
+                if constexpr (UPD) {
+                    const float xn = (float)(((double)yv[k] + pw) * pinv);
+                    if constexpr (MASS) tl.x[i] = xn;
+                    const float cv = dg[k] ? tl.damping * xn / (float)dg[k] : 0.0f;
+                    v[k] = Codec<FMT>::enc(cv * sc);
+                }
                 win[i] = v[k];
                 if constexpr (MASS) ms += (double)fabsf(Codec<FMT>::dec(v[k])) * (double)dg[k];
             }
@@ -608,13 +668,14 @@ __device__ __forceinline__ double cold_stage(const typename Codec<FMT>::store_t 
     return ms;
 }
 
-template <int FMT>
+template <int FMT, bool UPD>
 __global__ void __launch_bounds__(CW_THREADS)
 k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint32_t *__restrict__ tile_lo,
              const uint32_t *__restrict__ tcell, const uint16_t *__restrict__ wsrc, const uint32_t *wbase,
              const uint32_t *__restrict__ roff, uint32_t stage_max,
              const typename Codec<FMT>::store_t *__restrict__ contrib, typename Codec<FMT>::store_t *__restrict__ vals,
-             const uint32_t *__restrict__ deg, double *__restrict__ mass, const PrScalars *__restrict__ prs) {
+             const uint32_t *__restrict__ deg, double *__restrict__ mass, const PrScalars *__restrict__ prs,
+             PrTail tail) {
     using S = typename Codec<FMT>::store_t;
     __shared__ S win[COLD_TILE_V];
     __shared__ uint32_t dls[CW_DELTA_LDS];
@@ -626,8 +687,13 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     const uint32_t lo = tile_lo[t], w = min(tile_lo[t + 1] - lo, COLD_TILE_V);
     // the tile's first chunk: the mass of its edges, sum of |contribution|
     // x out-degree over its sources (a value is added once per edge)
-    const double ms0 = first ? cold_stage<FMT, true>(contrib + lo, deg + lo, w, win)
-                             : cold_stage<FMT, false>(contrib + lo, deg + lo, w, win);
+    PrTail tl = tail;
+    if constexpr (UPD) {
+        tl.y += lo;
+        tl.x += lo;
+    }
+    const double ms0 = first ? cold_stage<FMT, true, UPD>(contrib + lo, deg + lo, w, win, tl, prs)
+                             : cold_stage<FMT, false, UPD>(contrib + lo, deg + lo, w, win, tl, prs);
     const uint32_t o0 = tcell[t], nc = tcell[t + 1] - o0;   // the tile's non-empty cells
     const bool staged = nc <= stage_max;                     // their run offsets fit in LDS (stage_max <= CW_DELTA_LDS)
     if (staged)
@@ -914,14 +980,42 @@ k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, c
     block_emit<CC_THREADS>(my_sum, my_max, part, part_max, nbins + blockIdx.x, dred, fred);
 }
 
-int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
+// x' = (y + w) / s for the sources in no tile (no out-edges), whose
+// contributions no edge reads: the rest of the fused tail.
+__global__ void __launch_bounds__(256)
+k_pr_update_gaps(const uint32_t *__restrict__ gaps, uint32_t ngaps, const float *__restrict__ y, float *x,
+                 const PrScalars *__restrict__ prs) {
+    const uint32_t *pre = gaps + ngaps;
+    const uint32_t total = pre[ngaps];
+    const double w = prs->w, inv_s = prs->inv_s;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < total; i += gridDim.x * 256) {
+        uint32_t lo = 0, hi = ngaps;   // last r with pre[r] <= i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (pre[mid] <= i) lo = mid; else hi = mid;
+        }
+        const uint32_t v = gaps[lo] + (i - pre[lo]);
+        x[v] = (float)(((double)y[v] + w) * inv_s);
+    }
+}
+
+bool cold_tail_fusable(const gpr_graph *g) { return g->cold && g->cold->from == 0 && g->cold->ntiles > 0; }
+
+int cold_launch_write(gpr_graph *g, int fmt, const void *contrib, const float *y, float *x, float damping) {
     ColdLayout *L = g->cold;
     const uint32_t stage_max = (uint32_t)std::min<double>(CW_DELTA_LDS, tun().pr_write_stage_cells);
-#define CW(F)                                                                                                   \
-    hipLaunchKernelGGL(k_cold_write<F>, dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,           \
+    const PrTail tail = {y, x, damping};
+    const bool upd = x != nullptr;
+    if (upd && L->ngaps)
+        hipLaunchKernelGGL(k_pr_update_gaps, dim3(grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, L->gaps,
+                           L->ngaps, y, x, g->prs);
+#define CW_(F, U)                                                                                               \
+    hipLaunchKernelGGL((k_cold_write<F, U>), dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,      \
                        L->nwchunks, L->tile_lo, L->tcell, L->wsrc, L->wbase, L->roff, stage_max,                \
                        (const typename Codec<F>::store_t *)contrib, (typename Codec<F>::store_t *)L->vals, g->deg,  \
-                       L->mass, g->prs)
+                       L->mass, g->prs, tail)
+#define CW(F)                                                                                                   \
+    if (upd) CW_(F, true); else CW_(F, false)
     switch (fmt) {
     case GPR_CONTRIB_F32: CW(GPR_CONTRIB_F32); break;
     case GPR_CONTRIB_BF16: CW(GPR_CONTRIB_BF16); break;
@@ -931,6 +1025,7 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib) {
     case GPR_CONTRIB_SCFP16: CW(GPR_CONTRIB_SCFP16); break;
     }
 #undef CW
+#undef CW_
     GPR_HIP(hipGetLastError());
     return GPR_OK;
 }

@@ -271,6 +271,20 @@ int gpr_pr_finish(gpr_graph *g, gpr_prop *x, const gpr_prop *y,
                   double *sum_x);
 int gpr_pr_scalars(gpr_graph *g, double *sum_y, double *delta, double *sum_x,
                    float *enc_scale);
+/* One PageRank iteration with the previous iteration's vertex-map tail
+ * fused into this iteration's edgemap (no reference counterpart: PRv.C
+ * runs the maps of :214-269 after the edgemap of :202-212).  Equal to
+ * gpr_pr_finish(g, x, y, contrib, fmt, damping, NULL, NULL) followed by
+ * the GPR_RELAX_PR_SUM edgemap into y, except that the contributions are
+ * computed in the edgemap's write stage and never stored (a contribution
+ * property is left as it was) and no residual is computed (the device's
+ * delta and sum_x keep their values); x and y come out bit-identical to
+ * that sequence's.  For fixed iteration counts (no convergence test
+ * between iterations).  Needs every edge in the binned layout (tunable
+ * pr_split_bytes 0) and a graph with edges: GPR_E_INVAL otherwise.  st
+ * (optional) receives the step's stats. */
+int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt,
+                         float damping, gpr_iter_stats *st);
 
 /* ---- applications: the callers of the hot path -------------------------- */
 typedef struct {

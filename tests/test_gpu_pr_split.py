@@ -186,3 +186,66 @@ def test_write_run_offsets_from_hbm(name, cells):
         assert np.array_equal(xb, wantb)
     finally:
         G.tune_set("pr_write_stage_cells", old)
+
+
+@pytest.mark.parametrize("name", sorted(GRAPHS))
+@pytest.mark.parametrize("fmt", ["fp32", "bf16", "fp16", "cfp16", "scfp16"])
+def test_fused_tail_iteration_bit_identical(name, fmt):
+    """gpr_pr_iterate_fused equals gpr_pr_finish followed by the PR_SUM
+    edgemap, bit for bit in x and y, over three iterations from the same
+    state (every edge binned)."""
+    import ctypes as C
+    index, nbrs = GRAPHS[name]
+    f = G.CONTRIB_NAMES[fmt]
+    split_at(0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        n = g.n
+        runs = []
+        for fused in (False, True):
+            x, y = g.prop(G.F32), g.prop(G.F32)
+            cp = g.prop(G.U32 if fmt == "fp32" else G.U16)
+            x.upload(np.full(n, 1.0 / n, np.float32))
+            G._check(G.lib().gpr_pr_contrib(g.h, x.h, cp.h, f, C.c_float(0.85)))
+            g.edgemap(G.RELAX_PR_SUM, y, contrib=cp, contrib_fmt=f, damping=0.85)
+            for _ in range(3):
+                if fused:
+                    G._check(G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, f, C.c_float(0.85), None))
+                else:
+                    G._check(G.lib().gpr_pr_finish(g.h, x.h, y.h, cp.h, f, C.c_float(0.85), None, None))
+                    g.edgemap(G.RELAX_PR_SUM, y, contrib=cp, contrib_fmt=f, damping=0.85)
+            runs.append((x.download().view(np.uint32).copy(), y.download().view(np.uint32).copy()))
+            for p in (x, y, cp):
+                p.destroy()
+        assert np.array_equal(runs[0][0], runs[1][0]), (name, fmt, "x")
+        assert np.array_equal(runs[0][1], runs[1][1]), (name, fmt, "y")
+
+
+@pytest.mark.parametrize("name", ["rmat14", "grid", "star5000"])
+def test_prv_fused_tail_equals_unfused(name):
+    """PRv with a fixed iteration count (tail fused into the next write
+    stage) equals PRv with a convergence test that never stops it (tails
+    not fused): the same 20 iterations, bit-identical ranks."""
+    index, nbrs = GRAPHS[name]
+    split_at(0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        xf, inf = g.prv(max_iter=20, tol=0.0)
+        xu, inu = g.prv(max_iter=20, tol=1e-300)
+        assert inf.iters == inu.iters == 20
+        assert np.array_equal(xf.view(np.uint32), xu.view(np.uint32))
+        assert inf.delta == pytest.approx(inu.delta, rel=1e-12, abs=1e-300)
+
+
+def test_fused_tail_needs_binned_layout():
+    import ctypes as C
+    index, nbrs = GRAPHS["er"]
+    G.tune_set("pr_split_bytes", 1e12)   # every edge pulled: not fusable
+    with G.Graph.from_csr(index, nbrs) as g:
+        x, y = g.prop(G.F32), g.prop(G.F32)
+        rc = G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, G.CONTRIB_F32, C.c_float(0.85), None)
+        assert rc == G.GPR_E_INVAL
+        # PRv still runs (the tail is not fused) and matches the oracle
+        xr, info = g.prv(max_iter=20)
+        xc, _, _ = O.graptor_pr(index, nbrs, 20, 0.0)
+        assert np.abs(xr.astype(np.float64) - xc).sum() / np.abs(xc).sum() < 1e-6
+        x.destroy()
+        y.destroy()
