@@ -27,7 +27,7 @@ from . import TUNED, Graph, tune_get, tune_set
 
 CANDIDATES = {
     "pr_split_bytes": [0, 64 << 10, 256 << 10, 1 << 20, 2 << 20, 4 << 20, 8 << 20, 1e12],
-    "pull_serial_max": [8, 16, 32, 64, 128],
+    "pull_serial_max": [8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 65536],
     "pull_blocks_per_cu": [4, 8, 16, 32],
     "push_blocks_per_cu": [8, 32, 128],
 }

@@ -981,10 +981,12 @@ k_cold_combine(const uint32_t *__restrict__ multi, uint32_t nbins, uint32_t n, c
 }
 
 // x' = (y + w) / s for the sources in no tile (no out-edges), whose
-// contributions no edge reads: the rest of the fused tail.
+// contributions no edge reads: the rest of the fused tail.  zero_y: the
+// graph is symmetric, so these vertices have no in-edges either and y is 0
+// for them (y is never read: x' = (0 + w) / s, the same value).
 __global__ void __launch_bounds__(256)
 k_pr_update_gaps(const uint32_t *__restrict__ gaps, uint32_t ngaps, const float *__restrict__ y, float *x,
-                 const PrScalars *__restrict__ prs) {
+                 const PrScalars *__restrict__ prs, int zero_y) {
     const uint32_t *pre = gaps + ngaps;
     const uint32_t total = pre[ngaps];
     const double w = prs->w, inv_s = prs->inv_s;
@@ -995,7 +997,7 @@ k_pr_update_gaps(const uint32_t *__restrict__ gaps, uint32_t ngaps, const float 
             if (pre[mid] <= i) lo = mid; else hi = mid;
         }
         const uint32_t v = gaps[lo] + (i - pre[lo]);
-        x[v] = (float)(((double)y[v] + w) * inv_s);
+        x[v] = (float)(((zero_y ? 0.0 : (double)y[v]) + w) * inv_s);
     }
 }
 
@@ -1008,7 +1010,7 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib, const float *y
     const bool upd = x != nullptr;
     if (upd && L->ngaps)
         hipLaunchKernelGGL(k_pr_update_gaps, dim3(grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, L->gaps,
-                           L->ngaps, y, x, g->prs);
+                           L->ngaps, y, x, g->prs, g->symmetric ? 1 : 0);
 #define CW_(F, U)                                                                                               \
     hipLaunchKernelGGL((k_cold_write<F, U>), dim3(L->nwchunks), dim3(CW_THREADS), 0, g->stream, L->wchunk,      \
                        L->nwchunks, L->tile_lo, L->tcell, L->wsrc, L->wbase, L->roff, stage_max,                \
