@@ -91,10 +91,17 @@ __device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *
 // that a list of k entries takes about k / (cap - 64) atomics on that one
 // word instead of one per wave word.  The order of the entries in the
 // list is not specified.  All 64 lanes call push and flush together.
+//
+// With a limit (list_cap > 0), a wave that sees the list position pass it
+// stops queueing (full): the caller then takes another representation of
+// the same set (the list is incomplete), and the list costs at most about
+// list_cap / (cap - 64) + one atomic per wave.
 struct WaveQueue {
     uint32_t *q;     // this wave's cap entries of LDS
     uint32_t cap;
     uint32_t cnt;    // wave-uniform
+    uint32_t list_cap = 0;
+    bool full = false;
     __device__ __forceinline__ void flush(uint32_t *out, unsigned int *listpos) {
         if (!cnt) return;
         __builtin_amdgcn_wave_barrier();
@@ -103,12 +110,16 @@ struct WaveQueue {
         base = __shfl(base, 0, 64);
         for (uint32_t j = lane(); j < cnt; j += 64) out[base + j] = q[j];
         __builtin_amdgcn_wave_barrier();
+        if (list_cap && base + cnt > list_cap) full = true;
         cnt = 0;
     }
     __device__ __forceinline__ void push(bool keep, uint64_t word, uint32_t v, uint32_t *out,
                                          unsigned int *listpos) {
-        if (!word) return;
-        if (cnt + 64 > cap) flush(out, listpos);
+        if (!word || full) return;
+        if (cnt + 64 > cap) {
+            flush(out, listpos);
+            if (full) return;
+        }
         if (keep) q[cnt + mbcnt(word)] = v;
         cnt += (uint32_t)__popcll(word);
     }

@@ -96,48 +96,6 @@ template <int RELAX> __device__ __forceinline__ uint32_t relax_val(uint32_t ps) 
 // is set (unset: ~0); the value is the source's (internal) ID
 constexpr uint32_t SETIF_OPEN = 0x80000000u;
 
-// Fused policy inference: the first wave of the last workgroup to finish
-// turns the completed frontier counts into the next step's choice.  The
-// last one is found by tickets in two levels: a workgroup takes a ticket of
-// its counter stripe, and the last of a stripe one of the whole grid, so
-// no line takes more than one atomic per 64 workgroups (plus one per
-// stripe).  Each workgroup's counts are agent-scope atomics that its thread
-// 0 waits to be acknowledged (s_waitcnt vmcnt(0)) before its ticket, and
-// the last workgroup reads them with agent-scope loads: no fence, which on
-// this part would write back the XCD's L2 in every workgroup.  Each level's
-// last arrival resets its ticket for the next launch.
-__device__ __forceinline__ void policy_epilogue(const PolicyCtx &pol, DevCounters *ctr) {
-    if (!pol.params) return;
-    __shared__ bool last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-        const unsigned sidx = blockIdx.x % CTR_STRIPES;
-        const unsigned nstripes = gridDim.x < (unsigned)CTR_STRIPES ? gridDim.x : (unsigned)CTR_STRIPES;
-        const unsigned ssize = gridDim.x / CTR_STRIPES + (sidx < gridDim.x % CTR_STRIPES ? 1u : 0u);
-        CtrStripe *st = ctr_stripe(ctr);
-        bool l = false;
-        if (atomicAdd(&st->ticket, 1u) == ssize - 1) {
-            __hip_atomic_store(&st->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            l = atomicAdd(&ctr->ticket, 1u) == nstripes - 1;
-            if (l) __hip_atomic_store(&ctr->ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        last = l;
-    }
-    __syncthreads();
-    if (!last || threadIdx.x >= 64) return;   // the last workgroup's first wave
-    // the weights, then the striped counts (lane i reads stripe i): all
-    // loads in flight together
-    MlpWave net;
-    net.load(pol.params);
-    const CtrStripe *st = reinterpret_cast<const CtrStripe *>(ctr + 1) + lane();
-    const uint32_t nv = wave_sum(__hip_atomic_load(&st->nactv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    const uint64_t ne = wave_sum(__hip_atomic_load(&st->nacte, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    float f[6];
-    policy_features(pol, nv, ne, f);
-    const int d = net.run(f, pol.margin);
-    if (threadIdx.x == 0) ctr->decision = (unsigned)d;
-}
 
 // ---- push ------------------------------------------------------------------
 constexpr int PUSH_TILE = PUSH_THREADS * PUSH_ITEMS;
@@ -162,7 +120,7 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
        const uint64_t *__restrict__ btot, uint64_t E, const uint64_t *__restrict__ index,
        const uint32_t *__restrict__ nbrs, const uint32_t *__restrict__ deg, uint32_t *prop, int dkind,
        uint32_t darg, int record, const uint32_t *__restrict__ prev, uint64_t *bits, uint32_t *out_list,
-       DevCounters *ctr, PolicyCtx pol) {
+       DevCounters *ctr) {
     __shared__ uint64_t s_pos[PUSH_LDS];
     __shared__ uint32_t s_v[PUSH_LDS];
     __shared__ uint32_t q[PUSH_TILE];
@@ -256,7 +214,6 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
         if (tv) atomicAdd(&ctr_stripe(ctr)->nactv, tv);
         if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
     }
-    policy_epilogue(pol, ctr);
 }
 
 // ---- pull ------------------------------------------------------------------
@@ -275,7 +232,7 @@ __global__ void __launch_bounds__(PULL_THREADS)
 k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
        const uint32_t *__restrict__ deg, uint32_t *prop, const uint32_t *__restrict__ prev, int dkind,
        uint32_t darg, int record, const uint64_t *__restrict__ in_bits, uint64_t *out_bits,
-       uint32_t *out_list, DevCounters *ctr, PolicyCtx pol, uint32_t serial_max) {
+       uint32_t *out_list, DevCounters *ctr, uint32_t serial_max, uint32_t list_cap) {
     __shared__ uint64_t red[PULL_THREADS / 64];
     __shared__ uint32_t red32[PULL_THREADS / 64];
     extern __shared__ uint32_t pull_q[];   // list output: PULL_QCAP entries per wave
@@ -283,7 +240,7 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
     const uint32_t wid = (blockIdx.x * PULL_THREADS + threadIdx.x) / 64;
     const uint32_t nwaves = gridDim.x * (PULL_THREADS / 64);
     const unsigned ln = lane();
-    WaveQueue wq{pull_q + (threadIdx.x / 64) * PULL_QCAP, PULL_QCAP, 0};
+    WaveQueue wq{pull_q + (threadIdx.x / 64) * PULL_QCAP, PULL_QCAP, 0, list_cap};
     uint32_t my_v = 0;
     uint64_t my_e = 0, my_scan = 0;
     for (uint32_t grp = wid; grp < ngroups; grp += nwaves) {
@@ -379,7 +336,7 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         // workgroup (out_list is the same for the whole grid)
         __shared__ uint32_t qc[PULL_THREADS / 64], qbase;
         const unsigned w = threadIdx.x / 64;
-        if (ln == 0) qc[w] = wq.cnt;
+        if (ln == 0) qc[w] = wq.full ? 0u : wq.cnt;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t t = 0;
@@ -389,7 +346,8 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         __syncthreads();
         uint32_t off = qbase;
         for (unsigned j = 0; j < w; ++j) off += qc[j];
-        for (uint32_t j = ln; j < wq.cnt; j += 64) out_list[off + j] = wq.q[j];
+        if (!wq.full)
+            for (uint32_t j = ln; j < wq.cnt; j += 64) out_list[off + j] = wq.q[j];
     }
     const uint32_t tv = block_sum(my_v, red32);
     const uint64_t te = block_sum(my_e, red);
@@ -399,7 +357,6 @@ k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restri
         if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
         if (ts) atomicAdd(&ctr_stripe(ctr)->edges, (unsigned long long)ts);
     }
-    policy_epilogue(pol, ctr);
 }
 
 // ---- vertex maps -------------------------------------------------------------
@@ -586,6 +543,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     int okind = desc->out_kind;
     int fusion = desc->fusion;
     // trace runs force one step's action (no inference)
+    bool list_or_bits = false;   // pull: a list while it stays short, else the bitmap
     const bool forced = dir == GPR_DIR_POLICY && g->pol_force_step >= 0;
     const bool use_pol = dir == GPR_DIR_POLICY && (g->pol_params || forced);
     if (dir == GPR_DIR_POLICY) {
@@ -618,12 +576,15 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
             break;
         case 3:   // a pull the rule takes anyway, writing a list (a pull's
                   // cost grows with n, which the features do not carry, so
-                  // the network may not turn a sparse step into one)
+                  // the network may not turn a sparse step into one); the
+                  // bitmap is written too, and taken when the list would
+                  // be long (list_limit)
             if (threshold_sparse(desc->threshold_pct, nactv, nacte, m)) {
                 dir = GPR_DIR_AUTO;
             } else {
                 dir = GPR_DIR_PULL;
                 okind = GPR_FRONTIER_LIST;
+                list_or_bits = true;
             }
             break;
         default:  // the threshold rule's own step
@@ -639,8 +600,10 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     // (csr_sparse_no_f / csc_sparse_aset_no_f, api.h:2249-2252, 2321-2322)
     if (desc->record == GPR_RECORD_NONE) out = nullptr;
     if (!out) okind = 0;
+    if (!out) list_or_bits = false;
     if (okind == GPR_FRONTIER_LIST) GPR_TRY(frontier_ensure_list(out));
-    if (okind == GPR_FRONTIER_BITMAP) GPR_TRY(frontier_ensure_bits(out));
+    if (okind == GPR_FRONTIER_BITMAP || list_or_bits) GPR_TRY(frontier_ensure_bits(out));
+    const uint32_t list_limit = list_or_bits ? std::max<uint32_t>(65536u, n / 32) : 0u;
     uint32_t *prop = (uint32_t *)desc->prop->d;
     const uint32_t *prev = desc->prev ? (const uint32_t *)desc->prev->d : nullptr;
     GPR_REQUIRE(in || dir != GPR_DIR_PUSH,
@@ -712,7 +675,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
 #define PUSH(R)                                                                                                \
     hipLaunchKernelGGL(k_push<R>, dim3(grid), dim3(PUSH_THREADS), 0, g->stream, in->list, k, g->scan, g->scan_blk, E, \
                        g->index, g->nbrs, g->deg, prop, desc->dst_filter, desc->dst_arg, desc->record, prev, bits, \
-                       olist, g->ctr, pol)
+                       olist, g->ctr)
             if (desc->relax == GPR_RELAX_MIN_PLUS1) PUSH(GPR_RELAX_MIN_PLUS1);
             else if (desc->relax == GPR_RELAX_MIN) PUSH(GPR_RELAX_MIN);
             else PUSH(GPR_RELAX_SETIF);
@@ -721,15 +684,15 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         if (olist) GPR_TRY(clear_dedup_of_list(g, olist, E < n ? E : n));
         s.edges_scanned = E;
     } else {
-        uint64_t *ob = okind == GPR_FRONTIER_BITMAP ? out->bits : nullptr;
+        uint64_t *ob = okind == GPR_FRONTIER_BITMAP || list_or_bits ? out->bits : nullptr;
         uint32_t *ol = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
         int record = out ? desc->record : GPR_RECORD_NONE;
         ProfScope ps(g, "pull");
 #define PULL(R)                                                                                                 \
     hipLaunchKernelGGL(k_pull<R>, dim3(pull_grid(n)), dim3(PULL_THREADS), ol ? PULL_QBYTES : 0, g->stream, n,           \
                        g->cindex, g->cnbrs, g->deg,                                                              \
-                       prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol, g->ctr, pol,            \
-                       (uint32_t)tun().pull_serial_max)
+                       prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol, g->ctr,                 \
+                       (uint32_t)tun().pull_serial_max, list_limit)
         if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1);
         else if (desc->relax == GPR_RELAX_MIN) PULL(GPR_RELAX_MIN);
         else PULL(GPR_RELAX_SETIF);
@@ -737,7 +700,9 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
     }
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipEventRecord(g->ev1, g->stream));
-    GPR_TRY(read_counters(g));
+    // the counts' reduction also runs the network on them (the next step's
+    // choice), in the same one-wave kernel
+    GPR_TRY(read_counters(g, pol.params ? &pol : nullptr));
     float ms = 0.f;
     GPR_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
     if (desc->record == GPR_RECORD_NONE && out_arg) {
@@ -745,6 +710,7 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         out_arg->nactv = 0;
         out_arg->nacte = 0;
     }
+    if (list_or_bits && g->hctr->listpos > list_limit) okind = GPR_FRONTIER_BITMAP;   // the list is incomplete
     if (out) {
         out->kind = okind;
         out->nactv = okind == GPR_FRONTIER_LIST ? g->hctr->listpos : g->hctr->nactv;

@@ -315,7 +315,10 @@ int frontier_to_list(gpr_frontier *f);            // convert BITMAP/ALL -> LIST
 int fill_words(gpr_graph *g, uint64_t *p, size_t words, uint64_t v);
 int clear_dedup_of_list(gpr_graph *g, const uint32_t *list, uint64_t max_count);
 int ids_download(gpr_graph *g, const gpr_prop *p, uint32_t *host);   // ID-valued property, original IDs
-int read_counters(gpr_graph *g);                  // D2H of ctr, synchronises
+struct PolicyCtx;
+// totals of the counters into g->hctr (and with pol, the policy's next
+// choice into hctr->decision); synchronises
+int read_counters(gpr_graph *g, const PolicyCtx *pol = nullptr);
 int reset_counters(gpr_graph *g);
 int ensure_partials(gpr_graph *g, size_t count);
 int graph_setup(gpr_graph *g);                    // deg, buckets, long rows

@@ -5,9 +5,9 @@
 // nactv + nacte <= m/20, api.h:150-153; a list output for push and a dense
 // output for pull, dsl/edgemap.h:1571-1601) with the benchmark's own
 // fusion setting.  Learned policy: a 6-32-32-4 MLP (policy_dev.hpp) whose
-// forward pass runs in the last workgroup of each push/pull kernel; the
-// host reads its choice together with the frontier counts it already
-// reads (after a fused step, whose kernel does not run it, the host
+// forward pass runs in the one-wave reduction of each push/pull step's
+// frontier counts; the host reads its choice together with the counts it
+// already reads (after a fused step, which has no such reduction, the host
 // evaluates it).  Its four actions are the rule's own step, an unfused
 // push, a push fused while its levels stay small, and a pull with a list
 // output; another action replaces the rule's only with a logit lead of at

@@ -1,7 +1,8 @@
 // policy_dev.hpp -- the direction/representation policy network on the
-// device: a 6-32-32-4 MLP (ReLU, FP32) whose forward pass runs inside the
-// last workgroup of a traversal kernel, right after the frontier statistics
-// of that step are complete, so the next step's choice costs no launch.
+// device: a 6-32-32-4 MLP (ReLU, FP32) whose forward pass runs in the
+// one-wave kernel that reduces a step's frontier statistics (the striped
+// counters, runtime.hip k_read_counters), right after their totals, so the
+// next step's choice costs no launch of its own.
 //
 // Features (all of the frontier just produced; the edge counts relative
 // to m, as the threshold rule sees them, so that the features of a state

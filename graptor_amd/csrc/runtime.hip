@@ -2,6 +2,7 @@
 #include "internal.hpp"
 #include "device.hpp"
 #include "grid_sync.hpp"
+#include "policy_dev.hpp"
 
 #include <stdarg.h>
 #include <stdio.h>
@@ -56,24 +57,38 @@ int reset_counters(gpr_graph *g) {
     return GPR_OK;
 }
 
-// One wave sums the stripes (lane i reads stripe i) and writes the totals,
-// the list position and the policy decision to the host mirror.
-__global__ void __launch_bounds__(64) k_read_counters(const DevCounters *c, DevCounters *h) {
+// The frontier statistics' reduction: one wave sums the stripes (lane i
+// reads stripe i) and writes the totals and the list position to the host
+// mirror.  With a network (pol.params), the same wave then runs the
+// learned policy on the totals (policy_dev.hpp), its weights loaded beside
+// the stripes so that both arrive together, and writes the next step's
+// choice: the inference costs the step no launch, no host work and nothing
+// in the step's own kernels.
+__global__ void __launch_bounds__(64) k_read_counters(const DevCounters *c, DevCounters *h, PolicyCtx pol) {
+    MlpWave net;
+    if (pol.params) net.load(pol.params);
     const CtrStripe *st = reinterpret_cast<const CtrStripe *>(c + 1) + threadIdx.x;
     const unsigned long long ne = wave_sum(st->nacte), ed = wave_sum(st->edges);
     const unsigned int nv = wave_sum(st->nactv);
+    unsigned int decision = 0;
+    if (pol.params) {
+        float f[6];
+        policy_features(pol, nv, ne, f);
+        decision = (unsigned)net.run(f, pol.margin);
+    }
     if (threadIdx.x == 0) {
         h->nacte = ne;
         h->edges = ed;
         h->nactv = nv;
         h->listpos = c->listpos;
-        h->decision = c->decision;
+        h->decision = decision;
     }
 }
 
-int read_counters(gpr_graph *g) {
+int read_counters(gpr_graph *g, const PolicyCtx *pol) {
     static_assert(CTR_STRIPES == 64, "one stripe per lane of the read-back wave");
-    hipLaunchKernelGGL(k_read_counters, dim3(1), dim3(64), 0, g->stream, g->ctr, g->hctr_dev);
+    const PolicyCtx none = {};
+    hipLaunchKernelGGL(k_read_counters, dim3(1), dim3(64), 0, g->stream, g->ctr, g->hctr_dev, pol ? *pol : none);
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipStreamSynchronize(g->stream));
     return GPR_OK;

@@ -428,8 +428,8 @@ int gpr_batch_cc_noalloc(const uint64_t *d_index, const uint32_t *d_nbrs,
  * pulls (the rule's step where it pushes).  Features: log2 of the frontier
  * size, of its out-edges and of the unvisited edges (both relative to m),
  * the iteration, log mean degree and log degree skew.  Its forward pass
- * runs in the last workgroup of the step's kernel (on the host after a
- * fused step).  An action other than 0 is taken only when its logit leads
+ * runs on the device in the one-wave reduction of the step's frontier
+ * counts (on the host after a fused step).  An action other than 0 is taken only when its logit leads
  * action 0's by at least the margin.
  * Parameter layout: W1[6][32] b1[32] W2[32][32] b2[32] W3[32][4] b3[4]. */
 int gpr_mlp_param_count(void);

@@ -225,3 +225,26 @@ def test_trace_records_four_actions():
         assert len(f2) >= 2
         f3, t3 = g.policy_trace(0, s, cap=2)
         assert len(f3) <= 2
+
+
+def test_pull_list_falls_back_to_bitmap():
+    """Action 3 (a pull writing a list) keeps the bitmap it also writes
+    when the list would pass max(65536, n/32) entries: large outputs come
+    back as bitmaps, small ones as lists, and the labels stay exact."""
+    k = G.mlp_param_count()
+    index, nbrs = O.rmat_graph(18)
+    n = len(index) - 1
+    limit = max(65536, n // 32)
+    with G.Graph.from_csr(index, nbrs) as g:
+        want, _ = g.ccv()
+        p = np.zeros(k, np.float32)
+        p[k - OUT + 3] = 10.0
+        g.set_policy(p)
+        lab, _ = g.ccv(direction=G.DIR_POLICY, trace=True)
+        tr = g.trace()
+        g.set_policy(None)
+        assert np.array_equal(lab, want)
+        pulls = [t for t in tr if t["direction"] == G.DIR_PULL and t["nactv_out"]]
+        assert any(t["nactv_out"] > limit for t in pulls), tr
+        for t in pulls:
+            assert t["out_kind"] == (G.FRONTIER_BITMAP if t["nactv_out"] > limit else G.FRONTIER_LIST), t
