@@ -122,7 +122,7 @@ def sources(g, k=3, seed=7):
     return [g.highest_degree_vertex()] + [int(v) for v in g.order()[rng.choice(cand, k - 1, replace=False)]]
 
 
-def collect(graphs, cap=32):
+def collect(graphs, cap=64):
     """Traces of every graph: BFS from three sources and CC."""
     feats, times, tags = [], [], []
     for name, make in graphs:
@@ -240,7 +240,7 @@ def calibrate(p, graphs, margins=(0.0, 0.5, 1.0, 2.0, 3.0, 5.0, 8.0, 1e9), reps=
     return best["margin"], table
 
 
-def train(quick=False, epochs=4000, lr=3e-3, cap=32, out=WEIGHTS, seeds=(0, 1, 2, 3)):
+def train(quick=False, epochs=4000, lr=3e-3, cap=64, out=WEIGHTS, seeds=(0, 1, 2, 3)):
     """Trace, train one network per seed on the same traces, calibrate each
     one's gate on the validation split, and keep the network (and margin)
     with the best validation geomean among those whose worst validation run
@@ -294,7 +294,7 @@ def main():
     ap.add_argument("cmd", choices=["train", "eval"])
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--epochs", type=int, default=4000)
-    ap.add_argument("--cap", type=int, default=32)
+    ap.add_argument("--cap", type=int, default=64)
     ap.add_argument("--out", default=WEIGHTS)
     args = ap.parse_args()
     if args.cmd == "train":

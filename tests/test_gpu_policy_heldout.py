@@ -11,8 +11,8 @@ runs and the network's alternating.
 
 Checked: every output of the network's runs equals the rule's; one BFS per
 graph and the grid's CC also equal the oracle; the network is never more
-than 3% slower than the rule on any run, and at least 5% faster on the
-geometric mean (the committed network measures about 8.5%).
+than 3% slower than the rule on any run, and at least 10% faster on the
+geometric mean (the committed network measures about 1.19x).
 The measured figures are printed and written to gpurun_out/ when that
 folder exists.
 """
@@ -80,4 +80,4 @@ def test_heldout_speedup_and_bit_exact():
         with open(os.path.join(out, "policy_heldout.json"), "w") as fh:
             json.dump({"rows": rows, "geomean": geo, "worst": worst}, fh)
     assert worst >= 0.97
-    assert geo >= 1.05
+    assert geo >= 1.10
