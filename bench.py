@@ -51,15 +51,15 @@ def pr_pull_bytes(n, m, s):
 PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 
 
-# HBM bytes of one PR edgemap (FP32, every edge binned) on relabelled RMAT-26
-# ef16, from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the edgemap's
-# kernels (k_cold_write, k_cold_accum, k_cold_combine_groups,
+# HBM bytes of one PR edgemap (FP32, every edge binned, the previous
+# iteration's tail fused into its write stage) on relabelled RMAT-26 ef16,
+# from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the edgemap's kernels
+# (k_cold_write, k_pr_update_gaps, k_cold_accum, k_cold_combine_groups,
 # k_cold_combine).  Each counter is scaled by what it reads for
 # k_pr_update<fp32>, a plain stream of 12n bytes in and 8n out: FETCH_SIZE
-# x 2.00, WRITE_SIZE x 1.00.  The summary with the
-# commands is profiles/pr_edgemap_rmat26_pmc.md.  Other configurations
-# report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 12.385e9}
+# x 2.00, WRITE_SIZE x 1.00.  The summary with the commands is
+# profiles/pr_edgemap_rmat26_pmc.md.  Other configurations report null.
+MEASURED_TRAFFIC = {(26, 16, 0.0): 12.685e9}
 
 
 def run_bytes(trace, n, algo):
@@ -165,9 +165,15 @@ def run(args):
     b_cc = run_bytes(g.trace(), n, "cc")
     total_edges = eb + ec + ep
     value = total_edges / wall / 1e9
-    bytes_pull = pr_pull_bytes(n, m, 4)
-    achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
+    # with every edge binned, each iteration but the first carries the
+    # previous iteration's tail in its write stage (gpr_pr_iterate_fused):
+    # 12 B per vertex more (y and the degree read, x written), averaged
+    # over the profiled launches
+    bytes_edgemap = pr_pull_bytes(n, m, 4)
     split = G.tune_get("pr_split_bytes")
+    fused_share = (args.pr_iters - 1) / args.pr_iters if split == 0 and args.pr_iters > 0 else 0.0
+    bytes_pull = bytes_edgemap + 12 * n * fused_share
+    achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
     gteps = {
         "bfs": eb / (tb * 1e-3) / 1e9,
         "cc": ec / (tc * 1e-3) / 1e9,
@@ -206,6 +212,8 @@ def run(args):
             "traffic": (MEASURED_TRAFFIC.get((args.scale, args.edge_factor, split))
                         if not args.no_reorder else None),
             "algorithmic_bytes_per_launch": bytes_pull,
+            "edgemap_bytes": bytes_edgemap,
+            "fused_tail_bytes_per_vertex": 12 if fused_share else 0,
             "avg_launch_ms": avg_pull_ms,
             "kernels_avg_ms": per_scope,
             "launches": pull_launches,
