@@ -23,6 +23,7 @@ DST_NONE, DST_GT, DST_NE = range(3)
 RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE, RECORD_REDUCTION_OR_METHOD = range(4)
 DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
 CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T, CONTRIB_SCFP16 = range(6)
+PR_SKIP_UNLINKED = 1
 VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER, VMAP_COPY_FRONTIER = range(5)
 CONTRIB_NAMES = {"fp32": CONTRIB_F32, "bf16": CONTRIB_BF16, "fp16": CONTRIB_F16, "cfp16": CONTRIB_CFP16,
                  "bf16t": CONTRIB_BF16T, "scfp16": CONTRIB_SCFP16}
@@ -108,7 +109,7 @@ SIGNATURES = {
     "gpr_edgemap": (C.c_int, [_P, C.POINTER(EmapDesc), _P, _P, C.POINTER(IterStats)]),
     "gpr_vertex_map": (C.c_int, [_P, C.c_int, _P, _P, _P, C.c_uint64]),
     "gpr_vertex_filter": (C.c_int, [_P, _P, _P, C.c_uint32, C.c_int, _P]),
-    "gpr_pr_iterate_fused": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, _P]),
+    "gpr_pr_iterate_fused": (C.c_int, [_P, _P, _P, C.c_int, C.c_float, C.c_int, _P]),
     "gpr_pr_contrib": (C.c_int, [_P, _P, _P, C.c_int, C.c_float]),
     "gpr_pr_finish": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_float, C.POINTER(C.c_double),
                                 C.POINTER(C.c_double)]),

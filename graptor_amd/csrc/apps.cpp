@@ -362,7 +362,10 @@ struct PRv {
         while (iter < o.max_iter && sdelta > tol) {
             int rc = GPR_E_INVAL;
             if (pending) {
-                rc = gpr_pr_iterate_fused(GA.get(), x_mem.get(), y_mem.get(), fmt, dampen, nullptr);
+                // the sources without out-edges get their ranks in the last
+                // fused iteration only (nothing reads them before the final tail)
+                rc = gpr_pr_iterate_fused(GA.get(), x_mem.get(), y_mem.get(), fmt, dampen,
+                                          iter + 1 < o.max_iter ? GPR_PR_SKIP_UNLINKED : 0, nullptr);
                 if (rc == GPR_E_INVAL) {
                     // not fusable here: the deferred tail, then the plain edgemap
                     fuse_tail = false;

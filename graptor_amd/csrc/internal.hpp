@@ -228,6 +228,7 @@ struct gpr_graph {
     double pol_unvisited = 0;
     // trace runs: step pol_force_step takes action pol_force_action and every
     // other step the rule's; with pol_log set, each step's features are logged
+    bool pr_skip_unlinked = false;   // gpr_pr_iterate_fused with GPR_PR_SKIP_UNLINKED, for its call
     int pol_force_step = -1;
     int pol_force_action = 0;
     double pol_force_t0 = -1.0;   // host clock (ms) when the forced step began

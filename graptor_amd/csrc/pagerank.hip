@@ -469,7 +469,7 @@ int pr_edgemap(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_it
 
 using namespace gpr;
 
-extern "C" int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt, float damping,
+extern "C" int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt, float damping, int flags,
                                     gpr_iter_stats *st) {
     if (!g || !x || !y || x->dtype != GPR_F32 || y->dtype != GPR_F32 || x->g != g || y->g != g) return GPR_E_INVAL;
     GPR_REQUIRE(fmt >= GPR_CONTRIB_F32 && fmt <= GPR_CONTRIB_SCFP16, "pr_iterate_fused: unknown format %d", fmt);
@@ -487,7 +487,10 @@ extern "C" int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int 
     gpr_prop stand_in = *y;
     stand_in.elem = (uint32_t)fmt_bytes(fmt);
     d.contrib = &stand_in;
-    return pr_edgemap_impl(g, &d, nullptr, st, (float *)x->d);
+    g->pr_skip_unlinked = (flags & GPR_PR_SKIP_UNLINKED) != 0;
+    const int r = pr_edgemap_impl(g, &d, nullptr, st, (float *)x->d);
+    g->pr_skip_unlinked = false;
+    return r;
 }
 
 extern "C" int gpr_pr_contrib(gpr_graph *g, const gpr_prop *x, gpr_prop *contrib, int fmt, float damping) {

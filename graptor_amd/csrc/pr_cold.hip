@@ -1008,7 +1008,7 @@ int cold_launch_write(gpr_graph *g, int fmt, const void *contrib, const float *y
     const uint32_t stage_max = (uint32_t)std::min<double>(CW_DELTA_LDS, tun().pr_write_stage_cells);
     const PrTail tail = {y, x, damping};
     const bool upd = x != nullptr;
-    if (upd && L->ngaps)
+    if (upd && L->ngaps && !g->pr_skip_unlinked)
         hipLaunchKernelGGL(k_pr_update_gaps, dim3(grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, L->gaps,
                            L->ngaps, y, x, g->prs, g->symmetric ? 1 : 0);
 #define CW_(F, U)                                                                                               \

@@ -282,9 +282,14 @@ int gpr_pr_scalars(gpr_graph *g, double *sum_y, double *delta, double *sum_x,
  * that sequence's.  For fixed iteration counts (no convergence test
  * between iterations).  Needs every edge in the binned layout (tunable
  * pr_split_bytes 0) and a graph with edges: GPR_E_INVAL otherwise.  st
- * (optional) receives the step's stats. */
+ * (optional) receives the step's stats.  flags GPR_PR_SKIP_UNLINKED: the
+ * ranks of sources without out-edges are not written (their contributions
+ * are 0 and no edge reads them): x is then complete only after a later
+ * call without the flag, or gpr_pr_finish, updates them; a run of fused
+ * iterations passes it to all but its last. */
+enum { GPR_PR_SKIP_UNLINKED = 1 };
 int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt,
-                         float damping, gpr_iter_stats *st);
+                         float damping, int flags, gpr_iter_stats *st);
 
 /* ---- applications: the callers of the hot path -------------------------- */
 typedef struct {

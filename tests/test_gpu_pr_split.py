@@ -5,8 +5,8 @@ the edges from the other ("cold") sources through a binned layout with
 fixed-point accumulation (split point 0: every edge binned).  The split is
 set here per test (tunable pr_split_bytes = 4 bytes per pulled source;
 1e12 = no split) so that test-sized graphs take the binned path, with tiles
-capped by source count (32768; the edge cap of 2^24 per tile is above every
-graph here), several destination bins (16384 each), long rows (> 2048
+capped by source count (32768) and, in test_tiles_closed_by_edge_count, by
+edge count (2^24), several destination bins (16384 each), long rows (> 2048
 in-edges), directed graphs, and (on RMAT-20) hub bins of more than 2^20
 entries that are summed in chunks.
 
@@ -209,7 +209,7 @@ def test_fused_tail_iteration_bit_identical(name, fmt):
             g.edgemap(G.RELAX_PR_SUM, y, contrib=cp, contrib_fmt=f, damping=0.85)
             for _ in range(3):
                 if fused:
-                    G._check(G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, f, C.c_float(0.85), None))
+                    G._check(G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, f, C.c_float(0.85), 0, None))
                 else:
                     G._check(G.lib().gpr_pr_finish(g.h, x.h, y.h, cp.h, f, C.c_float(0.85), None, None))
                     g.edgemap(G.RELAX_PR_SUM, y, contrib=cp, contrib_fmt=f, damping=0.85)
@@ -241,7 +241,7 @@ def test_fused_tail_needs_binned_layout():
     G.tune_set("pr_split_bytes", 1e12)   # every edge pulled: not fusable
     with G.Graph.from_csr(index, nbrs) as g:
         x, y = g.prop(G.F32), g.prop(G.F32)
-        rc = G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, G.CONTRIB_F32, C.c_float(0.85), None)
+        rc = G.lib().gpr_pr_iterate_fused(g.h, x.h, y.h, G.CONTRIB_F32, C.c_float(0.85), 0, None)
         assert rc == G.GPR_E_INVAL
         # PRv still runs (the tail is not fused) and matches the oracle
         xr, info = g.prv(max_iter=20)
@@ -249,3 +249,29 @@ def test_fused_tail_needs_binned_layout():
         assert np.abs(xr.astype(np.float64) - xc).sum() / np.abs(xc).sum() < 1e-6
         x.destroy()
         y.destroy()
+
+
+def test_tiles_closed_by_edge_count():
+    """Sources whose edges fill a tile before its 32768-source cap: 8192
+    hubs of ~2200 edges each (18M hub edges, so a tile closes at ~7600
+    hubs by the 2^24-edge cap), joined to 200000 leaves.  Binning every
+    edge gives the pulled ranks (5e-8) and the FP64 ranks (1e-6), and runs
+    repeat bit for bit."""
+    hubs, leaves, per = 8192, 200000, 2200
+    rng = np.random.default_rng(24)
+    src = np.repeat(np.arange(hubs, dtype=np.uint32), per)
+    dst = (hubs + rng.integers(0, leaves, size=hubs * per)).astype(np.uint32)
+    index, nbrs = O.build_csr(hubs + leaves, src, dst, True)
+    deg = np.diff(index)
+    assert (deg[:hubs] > 0).all() and int(deg[:8000].sum()) > (1 << 24)
+    x64, _, _ = O.pr_fp64(index, nbrs, 20, 0.0)
+    with G.Graph.from_csr(index, nbrs) as g:
+        g.reorder()
+        G.tune_set("pr_split_bytes", 1e12)
+        x0, _ = g.prv(max_iter=20)
+        split_at(0)
+        x1, _ = g.prv(max_iter=20)
+        x2, _ = g.prv(max_iter=20)
+        assert np.array_equal(x1, x2)
+        assert rel_l1(x1, x0) <= 5e-8, rel_l1(x1, x0)
+        assert rel_l1(x1, x64) <= 1e-6, rel_l1(x1, x64)
