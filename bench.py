@@ -167,12 +167,19 @@ def run(args):
     value = total_edges / wall / 1e9
     # with every edge binned, each iteration but the first carries the
     # previous iteration's tail in its write stage (gpr_pr_iterate_fused):
-    # 12 B per vertex more (y and the degree read, x written), averaged
+    # 12 B per vertex updated (y and the degree read, x written): the
+    # vertices with out-edges, and in the last fused iteration also those
+    # without (their ranks are needed only by the final tail), averaged
     # over the profiled launches
     bytes_edgemap = pr_pull_bytes(n, m, 4)
     split = G.tune_get("pr_split_bytes")
-    fused_share = (args.pr_iters - 1) / args.pr_iters if split == 0 and args.pr_iters > 0 else 0.0
-    bytes_pull = bytes_edgemap + 12 * n * fused_share
+    index = np.empty(n + 1, np.uint64)
+    G._check(G.lib().gpr_graph_download(g.h, index.ctypes.data, None))
+    n_linked = int(np.count_nonzero(np.diff(index)))
+    del index
+    k = args.pr_iters
+    tail_bytes = 12 * ((k - 2) * n_linked + n) if split == 0 and k > 1 else 0
+    bytes_pull = bytes_edgemap + tail_bytes / max(1, k)
     achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
     gteps = {
         "bfs": eb / (tb * 1e-3) / 1e9,
@@ -213,7 +220,8 @@ def run(args):
                         if not args.no_reorder else None),
             "algorithmic_bytes_per_launch": bytes_pull,
             "edgemap_bytes": bytes_edgemap,
-            "fused_tail_bytes_per_vertex": 12 if fused_share else 0,
+            "fused_tail_bytes_per_launch": tail_bytes / max(1, k),
+            "vertices_with_out_edges": n_linked,
             "avg_launch_ms": avg_pull_ms,
             "kernels_avg_ms": per_scope,
             "launches": pull_launches,

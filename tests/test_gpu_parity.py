@@ -381,3 +381,50 @@ def test_rmat24_invariants():
         assert abs(float(np.sum(x, dtype=np.float64)) - 1.0) < 1e-5
         xc, _, _ = O.graptor_pr(index, nbrs, 3, 0.0)
         assert np.abs(x[order] - xc).sum() / xc.sum() < 1e-6
+
+
+def test_pull_list_output_queue_flushes():
+    """Pulls writing lists through the per-wave LDS queues, with a grid so
+    small (16 workgroups) that each wave processes ~64 groups of 64
+    destinations and flushes its queue mid-kernel (~1400 entries a wave in
+    the single step): BFS levels and CC labels
+    bit-exact, and a single pull step's list equals the set the rule
+    activates."""
+    index, nbrs = O.rmat_graph(18)
+    n = len(index) - 1
+    old = G.tune_get("pull_blocks_per_cu")
+    try:
+        G.tune_set("pull_blocks_per_cu", 16.0 / 256.0)
+        with G.Graph.from_csr(index, nbrs) as g:
+            s = O.highest_degree_vertex(index)
+            lv, _ = g.bfslvl(s, direction=G.DIR_PULL, out_kind=G.FRONTIER_LIST)
+            assert np.array_equal(lv, O.bfs_levels(index, nbrs, s))
+            lab, _ = g.ccv(direction=G.DIR_PULL, out_kind=G.FRONTIER_LIST)
+            assert np.array_equal(lab, O.cc_labels(index, nbrs))
+            # one pull step from a random 30% frontier (level 3) into
+            # unvisited destinations (level > 4): sources never change, so
+            # the step is the synchronous rule
+            rng = np.random.default_rng(18)
+            level = np.full(n, n + 1, np.uint32)
+            F = np.flatnonzero(rng.random(n) < 0.3).astype(np.uint32)
+            level[F] = 3
+            inF = np.zeros(n, bool)
+            inF[F] = True
+            hit = np.zeros(n, bool)
+            hit[nbrs[inF[edges_of(index)]]] = True
+            upd = hit & (level > 4)
+            exp = level.copy()
+            exp[upd] = 4
+            p = g.prop(G.U32)
+            p.upload(level)
+            fin = g.frontier().set_list(F)
+            fout = g.frontier()
+            st = g.edgemap(G.RELAX_MIN_PLUS1, p, fin, fout, dst_filter=G.DST_GT, dst_arg=4,
+                           direction=G.DIR_PULL, out_kind=G.FRONTIER_LIST)
+            assert np.array_equal(p.download(), exp)
+            assert st["out_kind"] == G.FRONTIER_LIST and st["nactv_out"] == int(upd.sum()) > 64 * 1024
+            assert np.array_equal(fout.download(), np.flatnonzero(upd).astype(np.uint32))
+            for f in (fin, fout, p):
+                f.destroy()
+    finally:
+        G.tune_set("pull_blocks_per_cu", old)
