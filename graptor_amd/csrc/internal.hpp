@@ -78,25 +78,26 @@ int num_cus();
 // The frontier counts are striped: workgroup b adds its totals to stripe
 // b % CTR_STRIPES (one 128-byte line each, right after the DevCounters
 // line), so no single line takes an atomic from every workgroup of a large
-// grid; the host sums the stripes when it reads the counters back, and
-// device readers sum them too.  The list append position stays one word
-// (it allocates positions).
+// grid; a one-wave kernel sums the stripes when the host reads the counters
+// back (k_read_counters).  The list append position stays one word (it
+// allocates positions).  The DevCounters line of the host mirror receives
+// the totals and the policy's choice.
 constexpr int CTR_STRIPES = 64;
 struct alignas(128) DevCounters {
-    unsigned long long nacte;   // edges of the frontier being produced (host: sum of the stripes)
-    unsigned long long edges;   // edges examined (host: sum of the stripes)
-    unsigned int nactv;         // vertices of the frontier being produced (host: sum of the stripes)
+    unsigned long long nacte;   // host mirror: edges of the frontier produced
+    unsigned long long edges;   // host mirror: edges examined
+    unsigned int nactv;         // host mirror: vertices of the frontier produced
     unsigned int listpos;       // append position of the list being produced
-    unsigned int ticket;        // last-workgroup tickets (reset by that wg)
-    unsigned int ticket2;
-    unsigned int decision;      // policy choice for the next step
+    unsigned int pad0;
+    unsigned int ticket2;       // last-workgroup ticket of the FP64 reductions (reset by that workgroup)
+    unsigned int decision;      // host mirror: the policy's choice for the next step
     unsigned int pad;
 };
 struct alignas(128) CtrStripe {
     unsigned long long nacte;
     unsigned long long edges;
     unsigned int nactv;
-    unsigned int ticket;        // workgroups of this stripe done (reset by its last)
+    unsigned int pad;
 };
 constexpr size_t COUNTER_BYTES = sizeof(DevCounters) + CTR_STRIPES * sizeof(CtrStripe);
 __device__ __forceinline__ CtrStripe *ctr_stripe(DevCounters *c) {

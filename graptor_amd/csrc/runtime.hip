@@ -47,7 +47,6 @@ __global__ void k_reset_counters(DevCounters *c) {
         st->nacte = 0;
         st->edges = 0;
         st->nactv = 0;
-        st->ticket = 0;
     }
 }
 
