@@ -62,7 +62,16 @@ k_scan_local(const uint32_t *__restrict__ list, uint32_t k, const uint32_t *__re
         if (i < k) pos[i] = run;
         run += d[j];
     }
-    if (threadIdx.x == SCAN_THREADS - 1) btot[blockIdx.x] = off + inc;
+    if (threadIdx.x == SCAN_THREADS - 1) {
+        if (gridDim.x == 1) {
+            // one block: its exclusive offset 0 and the total, as the top
+            // scan would write them (which then need not run)
+            btot[0] = 0;
+            btot[1] = off + inc;
+        } else {
+            btot[blockIdx.x] = off + inc;
+        }
+    }
 }
 
 // Exclusive scan of the block totals in place; total written to a[count].
@@ -425,7 +434,8 @@ k_vfilter_bits(uint32_t n, const uint32_t *__restrict__ p, uint32_t value, int n
 // keeps among its VF_TILE consecutive list entries (wave w takes the w-th
 // run of 64 * VF_ITEMS) and adds their number and out-degree sum to the
 // counters; after an exclusive scan of those counts, phase 1 writes the
-// kept entries at their offsets.  No shared append position is touched.
+// kept entries at their offsets (one workgroup: both in one launch).  No
+// shared append position is touched.
 constexpr int VF_THREADS = 256, VF_ITEMS = 8, VF_TILE = VF_THREADS * VF_ITEMS;
 __global__ void __launch_bounds__(VF_THREADS)
 k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__restrict__ p, uint32_t value,
@@ -451,7 +461,9 @@ k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__re
         cnt += (uint32_t)__popcll(m[it]);
     }
     if (ln == 0) wtot[w] = cnt;
-    if (phase == 0) {
+    if (phase != 1) {
+        // counting (phase 0; phase 2 = a single workgroup, which counts and
+        // writes at offset 0 in one launch)
         uint64_t es = 0;
 #pragma unroll
         for (int it = 0; it < VF_ITEMS; ++it)
@@ -460,14 +472,14 @@ k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__re
         if (threadIdx.x == 0) {
             uint32_t c = 0;
             for (int j = 0; j < VF_THREADS / 64; ++j) c += wtot[j];
-            blk[blockIdx.x] = c;
+            if (phase == 0) blk[blockIdx.x] = c;
             if (c) atomicAdd(&ctr_stripe(ctr)->nactv, c);
             if (te) atomicAdd(&ctr_stripe(ctr)->nacte, (unsigned long long)te);
         }
-        return;
+        if (phase == 0) return;
     }
     __syncthreads();
-    uint64_t off = blk[blockIdx.x];
+    uint64_t off = phase == 2 ? 0 : blk[blockIdx.x];
     for (unsigned j = 0; j < w; ++j) off += wtot[j];
 #pragma unroll
     for (int it = 0; it < VF_ITEMS; ++it) {
@@ -482,7 +494,7 @@ k_vfilter_list(const uint32_t *__restrict__ in, uint32_t k, const uint32_t *__re
 int scan_frontier_degrees(gpr_graph *g, const uint32_t *list, uint32_t k) {
     const uint32_t nb = (k + SCAN_BLOCK - 1) / SCAN_BLOCK;
     if (nb) hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, list, k, g->deg, g->scan, g->scan_blk);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
+    if (nb != 1) hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
 }
@@ -657,9 +669,10 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         const uint64_t E = in->nacte;
         const uint32_t nb = (k + SCAN_BLOCK - 1) / SCAN_BLOCK;
         ProfScope ps(g, "push");
-        hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, in->list, k, g->deg, g->scan,
-                           g->scan_blk);
-        hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
+        if (nb)
+            hipLaunchKernelGGL(k_scan_local, dim3(nb), dim3(SCAN_THREADS), 0, g->stream, in->list, k, g->deg,
+                               g->scan, g->scan_blk);
+        if (nb != 1) hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, g->stream, g->scan_blk, nb);
         // activations are deduplicated on the output bitmap, or for a list
         // output on the graph's dedup bitmap, which is cleared again from
         // the list afterwards; without an output nothing is activated, so
@@ -822,7 +835,10 @@ extern "C" int gpr_vertex_filter(gpr_graph *g, gpr_frontier *in, gpr_prop *p, ui
         const unsigned blocks = (unsigned)(((uint64_t)k + VF_TILE - 1) / VF_TILE);
         GPR_TRY(ensure_partials(g, blocks + 1));
         unsigned long long *cnt = (unsigned long long *)g->partial;
-        if (blocks) {
+        if (blocks == 1) {
+            hipLaunchKernelGGL(k_vfilter_list, dim3(1), dim3(VF_THREADS), 0, g->stream, in->list, k,
+                               (const uint32_t *)p->d, value, neg, out->list, g->deg, cnt, 2, g->ctr);
+        } else if (blocks) {
             hipLaunchKernelGGL(k_vfilter_list, dim3(blocks), dim3(VF_THREADS), 0, g->stream, in->list, k,
                                (const uint32_t *)p->d, value, neg, out->list, g->deg, cnt, 0, g->ctr);
             excl_scan_counts(g, cnt, blocks);

@@ -229,6 +229,7 @@ struct gpr_graph {
     double pol_unvisited = 0;
     // trace runs: step pol_force_step takes action pol_force_action and every
     // other step the rule's; with pol_log set, each step's features are logged
+    bool ctr_clean = true;           // counters zero (set by their read-back, cleared by a reset's kernels)
     bool pr_skip_unlinked = false;   // gpr_pr_iterate_fused with GPR_PR_SKIP_UNLINKED, for its call
     int pol_force_step = -1;
     int pol_force_action = 0;

@@ -50,7 +50,15 @@ __global__ void k_reset_counters(DevCounters *c) {
     }
 }
 
+// The read-back below leaves the counters zero, so a reset right after it
+// (the usual order: read at the end of one step, reset at the start of the
+// next) launches nothing.  g->ctr_clean: no kernel has touched the
+// counters since they were last zeroed.
 int reset_counters(gpr_graph *g) {
+    if (g->ctr_clean) {
+        g->ctr_clean = false;   // the caller's kernels follow
+        return GPR_OK;
+    }
     hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, g->stream, g->ctr);
     GPR_HIP(hipGetLastError());
     return GPR_OK;
@@ -58,17 +66,23 @@ int reset_counters(gpr_graph *g) {
 
 // The frontier statistics' reduction: one wave sums the stripes (lane i
 // reads stripe i) and writes the totals and the list position to the host
-// mirror.  With a network (pol.params), the same wave then runs the
+// mirror, and zeroes what it read for the next step.  With a network (pol.params), the same wave then runs the
 // learned policy on the totals (policy_dev.hpp), its weights loaded beside
 // the stripes so that both arrive together, and writes the next step's
 // choice: the inference costs the step no launch, no host work and nothing
 // in the step's own kernels.
-__global__ void __launch_bounds__(64) k_read_counters(const DevCounters *c, DevCounters *h, PolicyCtx pol) {
+__global__ void __launch_bounds__(64) k_read_counters(DevCounters *c, DevCounters *h, PolicyCtx pol) {
     MlpWave net;
     if (pol.params) net.load(pol.params);
-    const CtrStripe *st = reinterpret_cast<const CtrStripe *>(c + 1) + threadIdx.x;
-    const unsigned long long ne = wave_sum(st->nacte), ed = wave_sum(st->edges);
-    const unsigned int nv = wave_sum(st->nactv);
+    CtrStripe *st = reinterpret_cast<CtrStripe *>(c + 1) + threadIdx.x;
+    const unsigned long long e0 = st->nacte, e1 = st->edges;
+    const unsigned int v0 = st->nactv;
+    const unsigned int lp = c->listpos;
+    st->nacte = 0;
+    st->edges = 0;
+    st->nactv = 0;
+    const unsigned long long ne = wave_sum(e0), ed = wave_sum(e1);
+    const unsigned int nv = wave_sum(v0);
     unsigned int decision = 0;
     if (pol.params) {
         float f[6];
@@ -79,8 +93,9 @@ __global__ void __launch_bounds__(64) k_read_counters(const DevCounters *c, DevC
         h->nacte = ne;
         h->edges = ed;
         h->nactv = nv;
-        h->listpos = c->listpos;
+        h->listpos = lp;
         h->decision = decision;
+        c->listpos = 0;
     }
 }
 
@@ -90,6 +105,7 @@ int read_counters(gpr_graph *g, const PolicyCtx *pol) {
     hipLaunchKernelGGL(k_read_counters, dim3(1), dim3(64), 0, g->stream, g->ctr, g->hctr_dev, pol ? *pol : none);
     GPR_HIP(hipGetLastError());
     GPR_HIP(hipStreamSynchronize(g->stream));
+    g->ctr_clean = true;
     return GPR_OK;
 }
 
