@@ -52,14 +52,15 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 
 
 # HBM bytes of one PR edgemap (FP32, every edge binned, the previous
-# iteration's tail fused into its write stage) on relabelled RMAT-26 ef16,
+# iteration's tail fused into its write stage; averaged over a 20-iteration
+# run) on relabelled RMAT-26 ef16,
 # from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the edgemap's kernels
 # (k_cold_write, k_pr_update_gaps, k_cold_accum, k_cold_combine_groups,
 # k_cold_combine).  Each counter is scaled by what it reads for
 # k_pr_update<fp32>, a plain stream of 12n bytes in and 8n out: FETCH_SIZE
 # x 2.00, WRITE_SIZE x 1.00.  The summary with the commands is
 # profiles/pr_edgemap_rmat26_pmc.md.  Other configurations report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 12.685e9}
+MEASURED_TRAFFIC = {(26, 16, 0.0): 12.52e9}
 
 
 def run_bytes(trace, n, algo):
