@@ -308,14 +308,21 @@ struct CCv {
         }
         F.del();
         if (switched) {
-            // CCv.C:398-469, frac_threshold(200): always sparse
+            // CCv.C:398-469, frac_threshold(200): always sparse.  The
+            // reference filters each step's output for label 0 (and, in the
+            // second loop, for labels other than 0).  Here the output is
+            // taken as it is: a label only falls, to the minimum over
+            // sources of the frontier, so in the first loop (sources all
+            // labelled 0) every activated vertex ends at 0, and in the
+            // second (run once no vertex labelled 0 has a neighbour left
+            // that is not, and destinations labelled 0 filtered out) none
+            // does.  The filter would return its input, at the cost of a
+            // launch and a read-back per step.
             while (!zeros.isEmpty()) {
                 A::frontier Z = zeros;
-                A::frontier output = emap_step(Z, 200.f, info);
+                zeros = emap_step(Z, 200.f, info);
                 ++iter;
                 Z.del();
-                zeros = find_zeros(output);
-                output.del();
             }
             zeros.del();
             A::frontier all = A::frontier::all_true(GA);
@@ -323,11 +330,9 @@ struct CCv {
             all.del();
             while (!nonzeros.isEmpty()) {
                 A::frontier Z = nonzeros;
-                A::frontier output = emap_step(Z, 200.f, info);
+                nonzeros = emap_step(Z, 200.f, info);
                 ++iter;
                 Z.del();
-                nonzeros = find_nonzeros(output);
-                output.del();
             }
             nonzeros.del();
         }
