@@ -70,6 +70,7 @@ struct Tunables {
     double pr_write_stage_cells = 4096;       // PR write: a tile's run offsets go to LDS up to this many cells
     double fusion_small_limit = 1 << 17;      // GPR_FUSION_SMALL / _DEFAULT: frontier size that fuses / level size that ends it
     double fusion_cooperative = 1;            // fused push: 1 = hipLaunchCooperativeKernel, 0 = a plain launch
+    double push_bitmap_frac = 0.125;          // auto output of a push: a bitmap when the input's edges exceed this * n
 };
 Tunables &tun();
 int num_cus();
@@ -227,14 +228,18 @@ struct gpr_graph {
     int pol_next = 0;
     int pol_iter = 0;
     double pol_unvisited = 0;
-    // trace runs: step pol_force_step takes action pol_force_action and every
-    // other step the rule's; with pol_log set, each step's features are logged
+    // trace runs: step pol_force_step takes action pol_force_action (with
+    // pol_persist, so does every later step) and every other step the
+    // rule's (the loaded network's with pol_follow); with pol_log set, each
+    // step's features are logged
     bool ctr_clean = true;           // counters zero (set by their read-back, cleared by a reset's kernels)
     bool pr_skip_unlinked = false;   // gpr_pr_iterate_fused with GPR_PR_SKIP_UNLINKED, for its call
     int pol_force_step = -1;
     int pol_force_action = 0;
     double pol_force_t0 = -1.0;   // host clock (ms) when the forced step began
     bool pol_log = false;
+    bool pol_follow = false;
+    bool pol_persist = false;
     std::vector<float> pol_feat_log;
     uint32_t max_deg = 0;
     gpr::ColdLayout *cold = nullptr;   // PageRank cold-edge layout (built on first use)

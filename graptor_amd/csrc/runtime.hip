@@ -262,6 +262,7 @@ static double *tunable(const char *name) {
     if (!strcmp(name, "pr_write_stage_cells")) return &t.pr_write_stage_cells;
     if (!strcmp(name, "fusion_small_limit")) return &t.fusion_small_limit;
     if (!strcmp(name, "fusion_cooperative")) return &t.fusion_cooperative;
+    if (!strcmp(name, "push_bitmap_frac")) return &t.push_bitmap_frac;
     return nullptr;
 }
 

@@ -447,11 +447,19 @@ int gpr_policy_reset(gpr_graph *g);
 /* Training traces: algo 0 = BFSLVLv from start, 1 = CCv (min-label
  * propagation from all vertices).  The run is sampled at up to cap
  * iterations (evenly spread); for each sample: the 6 features and, for each
- * of the 4 actions, the measured ms (host clock, best of two runs) from
+ * of the 4 actions, the measured ms (host clock, best of three runs) from
  * the start of that step to the end of the run when the step takes the
- * action and every later step the threshold rule's
+ * action and every other step the threshold rule's
  * (BFSLVLv: no fusion; CCv: default_fusion_select, frac_threshold(5)).
+ * With GPR_TRACE_FOLLOW or'ed into algo, every other step is the loaded
+ * network's instead (states and costs-to-go under that network, for
+ * policy iteration); it needs a network (gpr_policy_set).  With
+ * GPR_TRACE_PERSIST, the forced action holds from that step to the end of
+ * the run (what a run of departures from the rule costs, which single-step
+ * departures do not show when each one pays a change of frontier
+ * representation).
  * *k receives the sample count. */
+enum { GPR_TRACE_FOLLOW = 0x100, GPR_TRACE_PERSIST = 0x200 };
 int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
                      float *features, float *times, int *k);
 /* The network on the GPU: logits of count samples; gradient of the
@@ -490,7 +498,11 @@ int gpr_trace_get(const gpr_graph *g, gpr_iter_stats *out, int cap, int *k);
  *                       offsets are staged in LDS when it has at most this
  *                       many non-empty (tile, bin) cells (capped at 4096,
  *                       the default), else read from HBM; the results are
- *                       the same either way */
+ *                       the same either way
+ *   push_bitmap_frac    out_kind 0: a push writes a bitmap (read back by
+ *                       the next step as an ID-sorted list) when its input
+ *                       frontier has more than this * n out-edges, else a
+ *                       list (default 0.125); a pull always a bitmap */
 int gpr_tune_set(const char *name, double value);
 int gpr_tune_get(const char *name, double *value);
 
