@@ -22,6 +22,8 @@ UNSET = 0xFFFFFFFF   # an unset BFSv parent
 DST_NONE, DST_GT, DST_NE = range(3)
 RECORD_NONE, RECORD_REDUCTION, RECORD_METHOD_NE, RECORD_REDUCTION_OR_METHOD = range(4)
 DIR_AUTO, DIR_PUSH, DIR_PULL, DIR_POLICY = range(4)
+TRACE_FOLLOW = 0x100    # gpr_policy_trace: the unforced steps follow the loaded network
+TRACE_PERSIST = 0x200   # gpr_policy_trace: the forced action holds to the end of the run
 CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T, CONTRIB_SCFP16 = range(6)
 PR_SKIP_UNLINKED = 1
 VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER, VMAP_COPY_FRONTIER = range(5)
@@ -382,14 +384,17 @@ class Graph:
         p = np.ascontiguousarray(params, np.float32)
         _check(lib().gpr_policy_set(self.h, _ptr(p), len(p)))
 
-    def policy_trace(self, algo, start=0, cap=64):
+    def policy_trace(self, algo, start=0, cap=64, follow=False, persist=False):
         """Features (k,6) of up to `cap` evenly spread states of a BFS
         (algo 0, from `start`) or CC (algo 1) run, and the ms to finish the
-        run from each state under each of the four actions (k,4)."""
+        run from each state under each of the four actions (k,4), every
+        other step taking the rule's action (follow=True: the loaded
+        network's; persist=True: the forced action holds to the end)."""
         f = np.empty((cap, 6), np.float32)
         t = np.empty((cap, 4), np.float32)
         k = C.c_int()
-        _check(lib().gpr_policy_trace(self.h, algo, start, cap, _ptr(f), _ptr(t), C.byref(k)))
+        flags = (TRACE_FOLLOW if follow else 0) | (TRACE_PERSIST if persist else 0)
+        _check(lib().gpr_policy_trace(self.h, algo | flags, start, cap, _ptr(f), _ptr(t), C.byref(k)))
         return f[: k.value].copy(), t[: k.value].copy()
 
     def trace(self):

@@ -382,21 +382,29 @@ extern "C" int gpr_mlp_train(float *params, const float *x, const float *targets
 
 // Trace generation: the benchmarks themselves (algo 0 = BFSLVLv from
 // `start`, 1 = CCv), each with its default settings.  A first run under
-// the rule logs every step's features and counts the steps; then, at up to
-// cap evenly spread steps i, the run is repeated with step i forced to each
-// action and every other step the rule's.  The runs share the steps before
-// i, so times[k][a] (the best of two runs' device time of run(), host round
-// trips included) differ by what action a at step i costs to the end.
+// the rule (with GPR_TRACE_FOLLOW, under the loaded network) logs every
+// step's features and counts the steps; then, at up to cap evenly spread
+// steps i, the run is repeated with step i forced to each action (with
+// GPR_TRACE_PERSIST, step i and every later step) and every other step the
+// rule's (the network's).  The runs share the steps before i, so
+// times[k][a] (the best of three runs' host-clock time from the start of
+// step i to the end) differ by what action a from step i on costs.
 extern "C" int gpr_bfslvl(gpr_graph *g, uint32_t start, const gpr_run_opts *o, uint32_t *levels, gpr_run_info *info);
 extern "C" int gpr_ccv(gpr_graph *g, const gpr_run_opts *o, uint32_t *labels, gpr_run_info *info);
 
 extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap, float *features, float *times,
                                 int *k) {
+    const bool follow = (algo & GPR_TRACE_FOLLOW) != 0;
+    const bool persist = (algo & GPR_TRACE_PERSIST) != 0;
+    algo &= ~(GPR_TRACE_FOLLOW | GPR_TRACE_PERSIST);
     if (!g || !k || cap <= 0 || (algo != 0 && algo != 1)) return GPR_E_INVAL;
     GPR_REQUIRE(algo == 1 || start < g->n, "trace: start out of range");
+    GPR_REQUIRE(!follow || !g->pol_host.empty(), "trace: GPR_TRACE_FOLLOW without a loaded network");
     *k = 0;
     float *saved = g->pol_params;
     g->pol_params = nullptr;
+    g->pol_follow = follow;
+    g->pol_persist = persist;
     gpr_run_opts o = {};
     o.direction = GPR_DIR_POLICY;
     o.max_iter = 100;
@@ -423,7 +431,7 @@ extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
     };
     int r = GPR_OK;
     do {
-        // the rule's run: features of every step
+        // the unforced run: features of every step
         double t0 = 0;
         g->pol_feat_log.clear();
         g->pol_log = true;
@@ -438,7 +446,7 @@ extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
             const int i = take == 1 ? 0 : (int)((int64_t)j * (steps - 1) / (take - 1));
             for (int a = 0; a < 4 && !r; ++a) {
                 double ms = 0;
-                if ((r = run(i, a, 2, &ms))) break;
+                if ((r = run(i, a, 3, &ms))) break;
                 if (times) times[j * 4 + a] = (float)ms;
             }
             if (features)
@@ -449,6 +457,8 @@ extern "C" int gpr_policy_trace(gpr_graph *g, int algo, uint32_t start, int cap,
     g->pol_force_step = -1;
     g->pol_force_action = 0;
     g->pol_log = false;
+    g->pol_follow = false;
+    g->pol_persist = false;
     g->pol_feat_log.clear();
     g->pol_params = saved;
     return r;

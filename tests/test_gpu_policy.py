@@ -227,6 +227,46 @@ def test_trace_records_four_actions():
         assert len(f3) <= 2
 
 
+def test_trace_follows_network():
+    """With follow=True the unforced steps are the loaded network's: the
+    sampled states are those of the network's own run (one per step while
+    the run has at most `cap` steps), not the rule's; without a network the
+    call is refused."""
+    from graptor_amd.policy import grid_csr
+    k = G.mlp_param_count()
+    index, nbrs = grid_csr(24, 24)   # BFS from a corner: 47 small levels
+    with G.Graph.from_csr(index, nbrs) as g:
+        s = 0
+        want, _ = g.bfslvl(s, trace=True)
+        rule_steps = len(g.trace())
+        assert rule_steps == 47
+        p = np.zeros(k, np.float32)
+        p[k - OUT + 2] = 10.0   # always a push fused while small
+        g.set_policy(p)
+        lv, _ = g.bfslvl(s, direction=G.DIR_POLICY, trace=True)
+        net_steps = len(g.trace())
+        assert np.array_equal(lv, want)
+        assert net_steps < rule_steps
+        f, t = g.policy_trace(0, s, follow=True)
+        assert len(f) == net_steps and (t > 0).all()
+        fr, _ = g.policy_trace(0, s)
+        assert len(fr) == rule_steps
+        assert np.array_equal(f[0], fr[0])   # the same first state
+        lv2, _ = g.bfslvl(s, direction=G.DIR_POLICY)   # the network stays loaded
+        assert np.array_equal(lv2, want)
+        # persist: the forced action holds to the end; the sampled states
+        # are still those of the unforced run
+        fp, tp = g.policy_trace(0, s, persist=True)
+        assert np.array_equal(fp, fr) and (tp > 0).all()
+        fpf, _ = g.policy_trace(0, s, follow=True, persist=True)
+        assert np.array_equal(fpf, f)
+        lv3, _ = g.bfslvl(s, direction=G.DIR_POLICY)
+        assert np.array_equal(lv3, want)
+        g.set_policy(None)
+        with pytest.raises(G.GprError):
+            g.policy_trace(0, s, follow=True)
+
+
 def test_pull_list_falls_back_to_bitmap():
     """Action 3 (a pull writing a list) keeps the bitmap it also writes
     when the list would pass max(65536, n/32) entries: large outputs come
