@@ -12,7 +12,7 @@ runs and the network's alternating.
 Checked: every output of the network's runs equals the rule's; one BFS per
 graph and the grid's CC also equal the oracle; the network is never more
 than 3% slower than the rule on any run, and at least 10% faster on the
-geometric mean (the committed network measures about 1.19x).
+geometric mean (the committed network measures about 1.13x).
 The measured figures are printed and written to gpurun_out/ when that
 folder exists.
 """
