@@ -303,6 +303,57 @@ def test_single_edgemap_step_matches_rule():
                 p.destroy()
 
 
+def test_push_tiles_over_empty_rows_and_auto_output():
+    """A push from a frontier that interleaves rows without edges with
+    ordinary ones (runs of equal offsets inside and across the 1024-edge
+    tiles, whose first and last sources a wave finds by 64-way search)
+    equals the CPU rule; with out_kind 0 the output is a bitmap exactly when
+    the input has more than push_bitmap_frac * n out-edges, and the same
+    vertices either way."""
+    index, nbrs = GRAPHS["rmat14"]
+    n = len(index) - 1
+    deg = np.diff(index)
+    rng = np.random.default_rng(21)
+    empty = np.flatnonzero(deg == 0)
+    assert len(empty) > 100
+    F = np.concatenate([rng.choice(empty, min(len(empty), 3000), replace=False),
+                        rng.integers(0, n, 3000)]).astype(np.uint32)
+    F = np.unique(F)
+    rng.shuffle(F)
+    level = np.full(n, n + 1, np.uint32)
+    level[F] = 3
+    src = edges_of(index)
+    inF = np.zeros(n, bool)
+    inF[F] = True
+    hit = np.zeros(n, bool)
+    hit[nbrs[inF[src]]] = True
+    upd = hit & (level > 4)
+    exp = level.copy()
+    exp[upd] = 4
+    nacte_in = int(deg[F].sum())
+    old = G.tune_get("push_bitmap_frac")
+    try:
+        for frac in (old, 0.0, 1e9, nacte_in / n * 0.999, nacte_in / n * 1.001):
+            G.tune_set("push_bitmap_frac", frac)
+            with G.Graph.from_csr(index, nbrs) as g:
+                p = g.prop(G.U32)
+                p.upload(level)
+                fin = g.frontier().set_list(F.copy())
+                fout = g.frontier()
+                st = g.edgemap(G.RELAX_MIN_PLUS1, p, fin, fout, dst_filter=G.DST_GT, dst_arg=4,
+                               direction=G.DIR_PUSH, out_kind=0)
+                assert np.array_equal(p.download(), exp), frac
+                assert np.array_equal(np.sort(fout.download()), np.flatnonzero(upd).astype(np.uint32)), frac
+                want = G.FRONTIER_BITMAP if nacte_in > frac * n else G.FRONTIER_LIST
+                assert st["out_kind"] == want, (frac, nacte_in, n)
+                assert st["nactv_out"] == upd.sum() and st["nacte_out"] == int(deg[upd].sum())
+                fin.destroy()
+                fout.destroy()
+                p.destroy()
+    finally:
+        G.tune_set("push_bitmap_frac", old)
+
+
 def test_vertex_filter_and_maps():
     index, nbrs = GRAPHS["er"]
     n = len(index) - 1
