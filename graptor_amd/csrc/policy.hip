@@ -1,7 +1,7 @@
 // policy.hip -- the per-iteration choice of traversal direction, output
 // frontier representation and fusion.
 //
-// Fallback and judge: the reference's threshold rule (sparse push iff
+// Fallback and baseline: the reference's threshold rule (sparse push iff
 // nactv + nacte <= m/20, api.h:150-153; a list output for push and a dense
 // output for pull, dsl/edgemap.h:1571-1601) with the benchmark's own
 // fusion setting.  Learned policy: a 6-32-32-4 MLP (policy_dev.hpp) whose

@@ -14,7 +14,7 @@
 //   f4 log2(1 + m/n) / 8                           mean degree
 //   f5 log2(1 + maxdeg/(m/n)) / 24                 degree skew
 // Outputs, the next step's action (argmax, gated):
-//   0 the threshold rule's own step (the judge: direction by the rule,
+//   0 the threshold rule's own step (the baseline: direction by the rule,
 //     the caller's fusion setting, the default output representation)
 //   1 an unfused sparse push (where the rule pulls, only while the
 //     frontier's edges stay within twice the rule's threshold)

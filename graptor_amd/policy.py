@@ -1,7 +1,7 @@
 """Training, gate calibration and evaluation of the learned policy.
 
 The policy picks each BFSLVLv / CCv step's action (policy_dev.hpp): 0 the
-threshold rule's own step (the judge), 1 an unfused push (where the rule
+threshold rule's own step (the baseline), 1 an unfused push (where the rule
 pulls, only up to twice its threshold), 2 a push fused while its levels
 stay small, 3 a pull writing a list where the rule pulls.  Pipeline (all
 on the GPU through the C ABI):
@@ -271,7 +271,7 @@ def train(quick=False, epochs=4000, lr=3e-3, cap=64, out=WEIGHTS, seeds=(0, 1, 2
     Policy iteration: the first traces follow the rule after the forced
     step, so their labels say what a step is worth when the rule takes
     over; a network that departs from the rule for several steps in a row
-    (pushing through a long mid-size phase, say) is judged one step at a
+    (pushing through a long mid-size phase, say) is evaluated one step at a
     time.  Each further iteration traces again with the kept network
     taking the unforced steps (its own states and costs-to-go), adds those
     samples, retrains, and the kept network is the best on the validation

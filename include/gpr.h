@@ -423,7 +423,7 @@ int gpr_batch_cc_noalloc(const uint64_t *d_index, const uint32_t *d_nbrs,
 
 /* ---- learned direction / representation policy (GPR_DIR_POLICY) --------
  * No counterpart in the reference, whose rule is the fixed threshold of
- * api.h:150-153 (the fallback when no network is loaded, and the judge).
+ * api.h:150-153 (the fallback when no network is loaded, and the baseline).
  * A 6-32-32-4 FP32 MLP picks each step's action: 0 = the threshold rule's
  * own step (direction by the rule, the caller's fusion setting and output
  * representation), 1 = an unfused sparse push (where the rule pulls, only
