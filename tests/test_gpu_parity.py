@@ -108,6 +108,53 @@ def test_cc_reordered_bit_exact(rgraph, d):
     assert np.array_equal(lab, O.cc_labels(index, nbrs)), (name, d)
 
 
+def zero_phase_graph():
+    """Vertex 0 at the end of a 100-vertex path (so that after CCv's first
+    step few vertices carry label 0 and the PUSH_ZERO loops run), beside a
+    60 x 60 grid, a 300-leaf star, an ER graph and isolated vertices: the
+    zero-label loop walks the path, the non-zero loop labels the rest."""
+    parts_s, parts_d = [], []
+    s = np.arange(99, dtype=np.uint32)
+    parts_s.append(s)
+    parts_d.append(s + 1)
+    gi, gn = O.grid_graph(60, 60)
+    gs = np.repeat(np.arange(len(gi) - 1), np.diff(gi).astype(np.int64)).astype(np.uint32)
+    parts_s.append(gs + 100)
+    parts_d.append(gn + 100)
+    leaves = np.arange(1, 301, dtype=np.uint32)
+    parts_s.append(np.full(300, 3700, np.uint32))
+    parts_d.append(leaves + 3700)
+    ei, en = O.er_graph(800, 4, seed=5)
+    es = np.repeat(np.arange(len(ei) - 1), np.diff(ei).astype(np.int64)).astype(np.uint32)
+    parts_s.append(es + 4001)
+    parts_d.append(en + 4001)
+    n = 4001 + 800 + 50   # 50 isolated vertices at the end
+    return O.build_csr(n, np.concatenate(parts_s), np.concatenate(parts_d))
+
+
+@pytest.mark.parametrize("reorder", [False, True])
+@pytest.mark.parametrize("d", DIRS)
+@pytest.mark.parametrize("deferred", [0, 1])
+@pytest.mark.parametrize("fusion", [0, 2, 3])
+def test_cc_push_zero_loops(reorder, d, deferred, fusion):
+    """CCv's PUSH_ZERO schedule (CCv.C:373-469) takes each step's output as
+    the next frontier in both of its loops (the reference filters it for
+    label 0, then for labels other than 0, and the filter returns its input
+    there); labels equal the CPU's on a graph that runs both loops."""
+    index, nbrs = zero_phase_graph()
+    want = O.cc_labels(index, nbrs)
+    with G.Graph.from_csr(index, nbrs) as g:
+        if reorder:
+            g.reorder()
+        lab, info = g.ccv(direction=d, deferred_update=deferred, fusion=fusion, trace=True)
+        assert np.array_equal(lab, want), (reorder, d, deferred, fusion)
+        if not reorder and fusion == 0 and d == G.DIR_AUTO:
+            # a pull over all vertices, then the path walked about a step
+            # per vertex by the zero-label loop (its steps are too small
+            # to fuse); a forced push fuses the first step instead
+            assert info.iters >= 90
+
+
 TOL = {"fp32": 1e-6, "bf16": 1e-3, "fp16": 1e-3, "cfp16": 1e-3, "bf16t": 1e-2}
 # The 16-bit error depends on the graph: on RMAT (skewed degrees) rounding
 # errors of many different contributions average out; on near-regular
