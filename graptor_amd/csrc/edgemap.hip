@@ -273,9 +273,12 @@ constexpr uint32_t PULL_SERIAL_MAX = 32;
 constexpr int PULL_UNROLL = 1;
 constexpr int PULL_HEAVY_UNROLL = 2;
 
-template <int RELAX>
+// IX: the CSC offsets' type (a 32-bit copy when m < 2^32: the pull reads
+// two offsets per destination it tests, a tenth of its first-step traffic
+// on RMAT-26 with 64-bit offsets)
+template <int RELAX, class IX>
 __global__ void __launch_bounds__(PULL_THREADS)
-k_pull(uint32_t n, const uint64_t *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
+k_pull(uint32_t n, const IX *__restrict__ cindex, const uint32_t *__restrict__ cnbrs,
        const uint32_t *__restrict__ deg, uint32_t *prop, const uint32_t *__restrict__ prev, int dkind,
        uint32_t darg, int record, const uint64_t *__restrict__ in_bits, uint64_t *out_bits,
        uint32_t *out_list, DevCounters *ctr, uint32_t serial_max, uint32_t list_cap) {
@@ -562,6 +565,26 @@ uint64_t threshold_limit(float pct, uint64_t m) {
     return (uint64_t)((double)(float)m / (100.0 / (double)pct));
 }
 
+// The CSC offsets as 32-bit values, when every offset fits (m < 2^32):
+// built once per CSR (graph_replace_csr drops the copy), on the graph's
+// stream; *out stays null otherwise.
+__global__ void k_narrow_offsets(const uint64_t *__restrict__ ix, uint32_t count, uint32_t *__restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+        out[i] = (uint32_t)ix[i];
+}
+static int csc_offsets32(gpr_graph *g, const uint32_t **out) {
+    *out = nullptr;
+    if (g->m >= (1ull << 32) || (uint64_t)g->n + 1 >= (1ull << 32)) return GPR_OK;
+    if (!g->cindex32) {
+        GPR_TRY(dev_alloc((void **)&g->cindex32, sizeof(uint32_t) * ((size_t)g->n + 1), "32-bit CSC index"));
+        hipLaunchKernelGGL(k_narrow_offsets, dim3(grid_for((uint64_t)g->n + 1, 256, 8192)), dim3(256), 0, g->stream,
+                           g->cindex, g->n + 1, g->cindex32);
+        GPR_HIP(hipGetLastError());
+    }
+    *out = g->cindex32;
+    return GPR_OK;
+}
+
 int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, gpr_frontier *out,
                      gpr_iter_stats *st) {
     gpr_frontier *const out_arg = out;
@@ -751,14 +774,22 @@ int edgemap_minrelax(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier *in, 
         uint32_t *ol = okind == GPR_FRONTIER_LIST ? out->list : nullptr;
         int record = out ? desc->record : GPR_RECORD_NONE;
         ProfScope ps(g, "pull");
-#define PULL(R)                                                                                                 \
-    hipLaunchKernelGGL(k_pull<R>, dim3(pull_grid(n)), dim3(PULL_THREADS), ol ? PULL_QBYTES : 0, g->stream, n,           \
-                       g->cindex, g->cnbrs, g->deg,                                                              \
+        const uint32_t *cix32 = nullptr;
+        GPR_TRY(csc_offsets32(g, &cix32));
+#define PULL(R, IX, CIX)                                                                                        \
+    hipLaunchKernelGGL((k_pull<R, IX>), dim3(pull_grid(n)), dim3(PULL_THREADS), ol ? PULL_QBYTES : 0, g->stream, n,     \
+                       CIX, g->cnbrs, g->deg,                                                                    \
                        prop, prev, desc->dst_filter, desc->dst_arg, record, in_bits, ob, ol, g->ctr,                 \
                        (uint32_t)tun().pull_serial_max, list_limit)
-        if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1);
-        else if (desc->relax == GPR_RELAX_MIN) PULL(GPR_RELAX_MIN);
-        else PULL(GPR_RELAX_SETIF);
+        if (cix32) {
+            if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1, uint32_t, cix32);
+            else if (desc->relax == GPR_RELAX_MIN) PULL(GPR_RELAX_MIN, uint32_t, cix32);
+            else PULL(GPR_RELAX_SETIF, uint32_t, cix32);
+        } else {
+            if (desc->relax == GPR_RELAX_MIN_PLUS1) PULL(GPR_RELAX_MIN_PLUS1, uint64_t, g->cindex);
+            else if (desc->relax == GPR_RELAX_MIN) PULL(GPR_RELAX_MIN, uint64_t, g->cindex);
+            else PULL(GPR_RELAX_SETIF, uint64_t, g->cindex);
+        }
 #undef PULL
     }
     GPR_HIP(hipGetLastError());

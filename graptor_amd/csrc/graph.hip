@@ -117,6 +117,7 @@ static void graph_free(gpr_graph *g) {
     if (!g) return;
     if (g->stream) hipStreamSynchronize(g->stream);
     prof_flush(g);
+    dev_free(g->cindex32);
     if (g->cindex != g->index) dev_free(g->cindex);
     if (g->cnbrs != g->nbrs) dev_free(g->cnbrs);
     dev_free(g->index);
@@ -212,6 +213,8 @@ uint32_t graph_max_degree(const gpr_graph *g) { return g->max_deg; }
 // rebuild the degree array and pull partitions; scratch stays.
 int graph_replace_csr(gpr_graph *g, uint64_t *index, uint32_t *nbrs) {
     GPR_HIP(hipStreamSynchronize(g->stream));
+    dev_free(g->cindex32);
+    g->cindex32 = nullptr;
     if (g->cindex != g->index) dev_free(g->cindex);
     if (g->cnbrs != g->nbrs) dev_free(g->cnbrs);
     dev_free(g->index);

@@ -186,6 +186,7 @@ struct gpr_graph {
     uint32_t *nbrs = nullptr;
     uint64_t *cindex = nullptr;   // == index when symmetric
     uint32_t *cnbrs = nullptr;
+    uint32_t *cindex32 = nullptr; // 32-bit copy of cindex for the pull when m < 2^32 (built on first use)
     uint32_t *deg = nullptr;
     // pull partitions over the CSC
     uint32_t nbuckets = 0;
