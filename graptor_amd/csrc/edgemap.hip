@@ -145,6 +145,52 @@ __device__ __forceinline__ uint32_t wave_ub_pos(const uint64_t *pos, const uint6
     return lo;
 }
 
+// First activation of d on the bitmap `bits` (the output or the dedup
+// bitmap): every lane that wants to record its destination sets d's bit,
+// and the lane whose set finds the bit clear records d.  Wave-uniform.
+// The edges of one source are consecutive lanes and its row is sorted, so
+// a hub's destinations come as runs of lanes on the same bitmap word; one
+// atomic per lane would queue on those words.  When the wave has such runs
+// (8 or more lanes joining the run of the lane before), each run ORs its
+// bits together (a segmented scan), its last lane sets them with one
+// atomic, and every lane of the run reads that atomic's old word; a lane
+// whose bit a lower lane of its run also holds leaves d to that lane.
+// Otherwise each lane sets its own bit.
+__device__ __forceinline__ bool bitmap_first(bool want, uint32_t d, uint64_t *bits) {
+    const uint32_t ln = lane();
+    const uint32_t key = want ? d >> 6 : 0xffffffffu - ln;   // idle lanes: a word of their own
+    const uint64_t bit = want ? 1ull << (d & 63) : 0ull;
+    const uint32_t pk = __shfl_up(key, 1, 64);
+    const bool head = ln == 0 || pk != key;
+    if (__popcll(__ballot(!head)) < 8) {
+        if (!want) return false;
+        const uint64_t ow = atomicOr((unsigned long long *)&bits[key], bit);
+        return !(ow & bit);
+    }
+    // inclusive OR within each run; flag: a run head lies in the lanes
+    // already combined
+    uint64_t inc = bit;
+    bool flag = head;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint64_t ob = __shfl_up(inc, off, 64);
+        const int of = __shfl_up((int)flag, off, 64);
+        if (ln >= (uint32_t)off && !flag) {
+            inc |= ob;
+            flag = of != 0;
+        }
+    }
+    const uint64_t before = __shfl_up(inc, 1, 64);
+    const uint64_t excl = head ? 0ull : before;
+    const uint32_t nk = __shfl_down(key, 1, 64);
+    const bool tail = ln == 63 || nk != key;
+    uint64_t old = 0;
+    if (tail && want) old = atomicOr((unsigned long long *)&bits[key], inc);
+    const uint64_t tails = __ballot(tail);
+    const int my_tail = __ffsll((long long)(tails & (~0ull << ln))) - 1;
+    const uint64_t ow = __shfl(old, my_tail, 64);
+    return want && !(ow & bit) && !(excl & bit);
+}
+
 // bits == nullptr: the step records nothing (no record argument, or no
 // output frontier); the relaxes still run.
 template <int RELAX>
@@ -189,7 +235,7 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
 #pragma unroll
         for (int it = 0; it < PUSH_ITEMS; ++it) {
             const uint64_t e = e0 + (uint64_t)it * PUSH_THREADS + threadIdx.x;
-            bool act = false;
+            bool want = false;   // the relax changed d and the step records it
             uint32_t d = 0;
             if (e < e1) {
                 uint64_t pj;
@@ -212,24 +258,17 @@ k_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__restric
                 if constexpr (RELAX == GPR_RELAX_SETIF) {
                     // single trigger: the CAS that replaces the open value
                     // wins, so each destination is set and activated once
-                    if ((pd & SETIF_OPEN) && dst_ok(pd, dkind, darg) && atomicCAS(&prop[d], pd, s) == pd &&
-                        bits && push_records(record, prop, prev, d)) {
-                        const uint64_t bit = 1ull << (d & 63);
-                        const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
-                        act = !(ow & bit);
-                    }
+                    want = (pd & SETIF_OPEN) && dst_ok(pd, dkind, darg) && atomicCAS(&prop[d], pd, s) == pd &&
+                           bits && push_records(record, prop, prev, d);
                 } else if (dst_ok(pd, dkind, darg)) {
                     const uint32_t nv = relax_val<RELAX>(prop[s]);
                     if (nv < pd) {
                         const uint32_t old = atomicMin(&prop[d], nv);
-                        if (old > nv && bits && push_records(record, prop, prev, d)) {
-                            const uint64_t bit = 1ull << (d & 63);
-                            const uint64_t ow = atomicOr((unsigned long long *)&bits[d >> 6], bit);
-                            act = !(ow & bit);
-                        }
+                        want = old > nv && bits && push_records(record, prop, prev, d);
                     }
                 }
             }
+            const bool act = bits ? bitmap_first(want, d, bits) : false;
             if (act) {
                 ++my_v;
                 my_e += deg[d];

@@ -31,12 +31,29 @@ __global__ void k_fill_all_bits(uint64_t *p, uint32_t n) {
     }
 }
 
+// One atomic per run of consecutive entries in the same bitmap word: a
+// push's list comes out in tile order, ascending within a tile, so the
+// neighbours of one source (a hub's million, on RMAT) share words, and one
+// atomic each would queue on those words.  The wave ORs each run's bits
+// together (a segmented scan over lanes whose word matches) and the run's
+// last lane stores them; a repeated word in another run is ORed twice,
+// which changes nothing.
 __global__ void k_list_to_bits(const uint32_t *__restrict__ list, uint32_t k, uint64_t *bits) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (; i < k; i += stride) {
-        uint32_t v = list[i];
-        atomicOr((unsigned long long *)&bits[v >> 6], 1ull << (v & 63));
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t ln = lane();
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < k; base += stride) {
+        const uint32_t i = base + ln;
+        const bool on = i < k;
+        const uint32_t v = on ? list[i] : 0u;
+        const uint32_t w = on ? v >> 6 : 0xffffffffu - ln;   // idle lanes: a word of their own
+        uint64_t b = on ? 1ull << (v & 63) : 0ull;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint64_t ob = __shfl_up(b, off, 64);
+            const uint32_t ow = __shfl_up(w, off, 64);
+            if (ln >= (uint32_t)off && ow == w) b |= ob;
+        }
+        const uint32_t nw = __shfl_down(w, 1, 64);
+        if (on && (ln == 63 || nw != w)) atomicOr((unsigned long long *)&bits[w], b);
     }
 }
 
