@@ -93,11 +93,14 @@ k_apr_push(const uint32_t *__restrict__ flist, uint32_t k, const uint64_t *__res
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t e0 = t * AP_TILE;
         const uint64_t e1 = min(E, e0 + AP_TILE);
-        if (threadIdx.x == 0) {
-            const uint32_t i0 = ub_pos(pos, btot, k, e0) - 1;
-            const uint32_t i1 = ub_pos(pos, btot, k, e1 - 1) - 1;
-            s_i0 = i0;
-            s_cnt = i1 - i0 + 1;
+        if (threadIdx.x < 64) {
+            // the tile's first and last sources, by one wave's 64-way search
+            const uint32_t i0 = wave_ub_pos(pos, btot, k, e0, 0u) - 1;
+            const uint32_t i1 = wave_ub_pos(pos, btot, k, e1 - 1, i0 + 1) - 1;
+            if (threadIdx.x == 0) {
+                s_i0 = i0;
+                s_cnt = i1 - i0 + 1;
+            }
         }
         __syncthreads();
         const uint32_t i0 = s_i0, cnt = s_cnt;

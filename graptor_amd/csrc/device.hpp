@@ -84,6 +84,30 @@ __device__ __forceinline__ uint32_t ub_pos(const uint64_t *pos, const uint64_t *
     return lo;
 }
 
+// First index i in [lo, k) with pos(i) > e (k if none), by one wave: 64
+// probes per round, evenly spaced over the remaining range, and a ballot of
+// pos <= e (a prefix of the lanes, pos being non-decreasing) narrows the
+// range 64-fold; about log64(k) rounds of one load each, where a one-lane
+// binary search waits on log2(k) loads in a row.  All 64 lanes take part
+// and get the result.
+__device__ __forceinline__ uint32_t wave_ub_pos(const uint64_t *pos, const uint64_t *btot, uint32_t k, uint64_t e,
+                                                uint32_t lo) {
+    uint32_t hi = k;   // the answer lies in [lo, hi]
+    const uint32_t l = lane();
+    while (lo < hi) {
+        const uint32_t step = (hi - lo + 63) / 64;
+        const uint32_t p = lo + l * step;
+        const bool le = p < hi && frontier_pos(pos, btot, p) <= e;
+        const uint32_t c = (uint32_t)__popcll(__ballot(le));
+        if (c == 0) break;   // pos(lo) > e
+        // the answer lies in (lo + (c-1) step, lo + c step], clipped to hi
+        const uint32_t nhi = min(lo + c * step, hi);
+        lo = lo + (c - 1) * step + 1;
+        hi = nhi;
+    }
+    return lo;
+}
+
 // A wave's private queue of list entries in LDS (cap entries, cap a
 // multiple of 64 and >= 128).  Entries go in 64 lanes at a time, as a
 // ballot word; the queue is appended to the output list with ONE atomic on

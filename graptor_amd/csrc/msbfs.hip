@@ -61,21 +61,14 @@ k_ms_push(const uint32_t *__restrict__ flist, uint32_t kf, const uint64_t *__res
     const uint64_t ntiles = (E + MS_TILE - 1) / MS_TILE;
     for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const uint64_t e0 = t * MS_TILE, e1 = min(E, e0 + MS_TILE);
-        if (threadIdx.x == 0) {
-            uint32_t lo = 0, hi = kf;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (pos[mid] + btot[mid / SCAN_BLOCK] <= e0) lo = mid + 1; else hi = mid;
+        if (threadIdx.x < 64) {
+            // the tile's first and last sources, by one wave's 64-way search
+            const uint32_t i0 = wave_ub_pos(pos, btot, kf, e0, 0u) - 1;
+            const uint32_t i1 = wave_ub_pos(pos, btot, kf, e1 - 1, i0 + 1);
+            if (threadIdx.x == 0) {
+                s_i0 = i0;
+                s_cnt = i1 - i0;
             }
-            const uint32_t i0 = lo - 1;
-            lo = i0;
-            hi = kf;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (pos[mid] + btot[mid / SCAN_BLOCK] <= e1 - 1) lo = mid + 1; else hi = mid;
-            }
-            s_i0 = i0;
-            s_cnt = lo - i0;
         }
         __syncthreads();
         const uint32_t i0 = s_i0, cnt = s_cnt;
