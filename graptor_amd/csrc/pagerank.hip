@@ -217,6 +217,10 @@ __global__ void k_pr_long(uint32_t nlong, const uint32_t *__restrict__ long_rows
 // maximum only.  mode 3: the cold-edge mass -- set the fixed-point unit.
 constexpr int RED_THREADS = 256;
 constexpr int RED_BLOCKS = 64;
+// Workgroups of a reduction: one for a few thousand partials (a launch of
+// 64 and the last-workgroup fold cost more than the sums), RED_BLOCKS above
+// that; the sums stay deterministic for a given count
+static unsigned red_grid(uint32_t count) { return count <= 16384u ? 1u : (unsigned)RED_BLOCKS; }
 __global__ void __launch_bounds__(RED_THREADS)
 k_pr_reduce(const double *__restrict__ part, const float *__restrict__ part_max, const double *__restrict__ part2,
             uint32_t count, double *slices, unsigned int *ticket, PrScalars *prs, int mode, uint32_t n,
@@ -405,7 +409,7 @@ static int pr_edgemap_impl(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier
     if (split) {
         ProfScope ps(g, "pr_cold_write");
         GPR_TRY(cold_launch_write(g, fmt, cb, y, x_tail, damping));
-        hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, g->cold->mass,
+        hipLaunchKernelGGL(k_pr_reduce, dim3(red_grid(g->cold->nwchunks)), dim3(RED_THREADS), 0, g->stream, g->cold->mass,
                            (const float *)nullptr, (const double *)nullptr, g->cold->nwchunks, g->dsum_slices,
                            &g->ctr->ticket2, g->prs, 3, g->n, damping, 0);
     }
@@ -440,7 +444,7 @@ static int pr_edgemap_impl(gpr_graph *g, const gpr_emap_desc *desc, gpr_frontier
     }
     {
         ProfScope ps(g, "pr_reduce");
-        hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, g->partial, g->partial_f,
+        hipLaunchKernelGGL(k_pr_reduce, dim3(red_grid(nparts)), dim3(RED_THREADS), 0, g->stream, g->partial, g->partial_f,
                            (const double *)nullptr, nparts, g->dsum_slices, &g->ctr->ticket2, g->prs, 0, g->n,
                            damping, fmt == GPR_CONTRIB_F16);
     }
@@ -549,7 +553,7 @@ extern "C" int gpr_pr_finish(gpr_graph *g, gpr_prop *x, const gpr_prop *y, gpr_p
         case GPR_CONTRIB_SCFP16: PR_UPDATE(GPR_CONTRIB_SCFP16); break;
         }
 #undef PR_UPDATE
-        hipLaunchKernelGGL(k_pr_reduce, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, g->stream, pd, g->partial_f, psum, grid,
+        hipLaunchKernelGGL(k_pr_reduce, dim3(red_grid(grid)), dim3(RED_THREADS), 0, g->stream, pd, g->partial_f, psum, grid,
                            g->dsum_slices, &g->ctr->ticket2, g->prs, 1, n, damping, 0);
     }
     GPR_HIP(hipGetLastError());
