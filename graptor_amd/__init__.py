@@ -26,6 +26,7 @@ TRACE_FOLLOW = 0x100    # gpr_policy_trace: the unforced steps follow the loaded
 TRACE_PERSIST = 0x200   # gpr_policy_trace: the forced action holds to the end of the run
 CONTRIB_F32, CONTRIB_BF16, CONTRIB_F16, CONTRIB_CFP16, CONTRIB_BF16T, CONTRIB_SCFP16 = range(6)
 PR_SKIP_UNLINKED = 1
+PR_SKIP_RANKS = 2
 VMAP_FILL, VMAP_COPY, VMAP_IOTA, VMAP_FILL_FRONTIER, VMAP_COPY_FRONTIER = range(5)
 CONTRIB_NAMES = {"fp32": CONTRIB_F32, "bf16": CONTRIB_BF16, "fp16": CONTRIB_F16, "cfp16": CONTRIB_CFP16,
                  "bf16t": CONTRIB_BF16T, "scfp16": CONTRIB_SCFP16}

@@ -370,7 +370,8 @@ struct PRv {
                 // the sources without out-edges get their ranks in the last
                 // fused iteration only (nothing reads them before the final tail)
                 rc = gpr_pr_iterate_fused(GA.get(), x_mem.get(), y_mem.get(), fmt, dampen,
-                                          iter + 1 < o.max_iter ? GPR_PR_SKIP_UNLINKED : 0, nullptr);
+                                          iter + 1 < o.max_iter ? GPR_PR_SKIP_UNLINKED | GPR_PR_SKIP_RANKS : 0,
+                                          nullptr);
                 if (rc == GPR_E_INVAL) {
                     // not fusable here: the deferred tail, then the plain edgemap
                     fuse_tail = false;

@@ -235,6 +235,7 @@ struct gpr_graph {
     // step's features are logged
     bool ctr_clean = true;           // counters zero (set by their read-back, cleared by a reset's kernels)
     bool pr_skip_unlinked = false;   // gpr_pr_iterate_fused with GPR_PR_SKIP_UNLINKED, for its call
+    bool pr_skip_ranks = false;      // gpr_pr_iterate_fused with GPR_PR_SKIP_RANKS, for its call
     int pol_force_step = -1;
     int pol_force_action = 0;
     double pol_force_t0 = -1.0;   // host clock (ms) when the forced step began

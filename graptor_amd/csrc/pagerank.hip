@@ -491,9 +491,11 @@ extern "C" int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int 
     gpr_prop stand_in = *y;
     stand_in.elem = (uint32_t)fmt_bytes(fmt);
     d.contrib = &stand_in;
-    g->pr_skip_unlinked = (flags & GPR_PR_SKIP_UNLINKED) != 0;
+    g->pr_skip_unlinked = (flags & (GPR_PR_SKIP_UNLINKED | GPR_PR_SKIP_RANKS)) != 0;
+    g->pr_skip_ranks = (flags & GPR_PR_SKIP_RANKS) != 0;
     const int r = pr_edgemap_impl(g, &d, nullptr, st, (float *)x->d);
     g->pr_skip_unlinked = false;
+    g->pr_skip_ranks = false;
     return r;
 }
 

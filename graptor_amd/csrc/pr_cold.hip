@@ -656,7 +656,9 @@ __device__ __forceinline__ double cold_stage(const typename Codec<FMT>::store_t 
             if (i < w) {
                 if constexpr (UPD) {
                     const float xn = (float)(((double)yv[k] + pw) * pinv);
-                    if constexpr (MASS) tl.x[i] = xn;
+                    if constexpr (MASS) {
+                        if (tl.x) tl.x[i] = xn;
+                    }
                     const float cv = dg[k] ? tl.damping * xn / (float)dg[k] : 0.0f;
                     v[k] = Codec<FMT>::enc(cv * sc);
                 }
@@ -690,7 +692,7 @@ k_cold_write(const uint32_t *__restrict__ wchunk, uint32_t nwchunks, const uint3
     PrTail tl = tail;
     if constexpr (UPD) {
         tl.y += lo;
-        tl.x += lo;
+        if (tl.x) tl.x += lo;
     }
     const double ms0 = first ? cold_stage<FMT, true, UPD>(contrib + lo, deg + lo, w, win, tl, prs)
                              : cold_stage<FMT, false, UPD>(contrib + lo, deg + lo, w, win, tl, prs);
@@ -1006,7 +1008,9 @@ bool cold_tail_fusable(const gpr_graph *g) { return g->cold && g->cold->from == 
 int cold_launch_write(gpr_graph *g, int fmt, const void *contrib, const float *y, float *x, float damping) {
     ColdLayout *L = g->cold;
     const uint32_t stage_max = (uint32_t)std::min<double>(CW_DELTA_LDS, tun().pr_write_stage_cells);
-    const PrTail tail = {y, x, damping};
+    // with GPR_PR_SKIP_RANKS the tail computes the contributions but writes
+    // no rank (x stays null in the kernel)
+    const PrTail tail = {y, g->pr_skip_ranks ? nullptr : x, damping};
     const bool upd = x != nullptr;
     if (upd && L->ngaps && !g->pr_skip_unlinked)
         hipLaunchKernelGGL(k_pr_update_gaps, dim3(grid_for(g->n, 256, 2048)), dim3(256), 0, g->stream, L->gaps,

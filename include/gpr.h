@@ -286,8 +286,12 @@ int gpr_pr_scalars(gpr_graph *g, double *sum_y, double *delta, double *sum_x,
  * ranks of sources without out-edges are not written (their contributions
  * are 0 and no edge reads them): x is then complete only after a later
  * call without the flag, or gpr_pr_finish, updates them; a run of fused
- * iterations passes it to all but its last. */
-enum { GPR_PR_SKIP_UNLINKED = 1 };
+ * iterations passes it to all but its last.  GPR_PR_SKIP_RANKS: no rank is
+ * written (the next fused call computes its contributions from y, not
+ * from x), which implies GPR_PR_SKIP_UNLINKED: x keeps its old values until
+ * a later call without either flag; a run of fused iterations passes both
+ * to all but its last. */
+enum { GPR_PR_SKIP_UNLINKED = 1, GPR_PR_SKIP_RANKS = 2 };
 int gpr_pr_iterate_fused(gpr_graph *g, gpr_prop *x, gpr_prop *y, int fmt,
                          float damping, int flags, gpr_iter_stats *st);
 
