@@ -60,7 +60,7 @@ PR_SCOPES = ("pr_cold_write", "pr_pull", "pr_long", "pr_cold_accum")
 # k_pr_update<fp32>, a plain stream of 12n bytes in and 8n out: FETCH_SIZE
 # x 2.00, WRITE_SIZE x 1.00.  The summary with the commands is
 # profiles/pr_edgemap_rmat26_pmc.md.  Other configurations report null.
-MEASURED_TRAFFIC = {(26, 16, 0.0): 12.52e9}
+MEASURED_TRAFFIC = {(26, 16, 0.0): 12.43e9}
 
 
 def run_bytes(trace, n, algo):
@@ -168,10 +168,10 @@ def run(args):
     value = total_edges / wall / 1e9
     # with every edge binned, each iteration but the first carries the
     # previous iteration's tail in its write stage (gpr_pr_iterate_fused):
-    # 12 B per vertex updated (y and the degree read, x written): the
-    # vertices with out-edges, and in the last fused iteration also those
-    # without (their ranks are needed only by the final tail), averaged
-    # over the profiled launches
+    # 8 B per vertex with out-edges (y and the degree read; the ranks are
+    # not written, GPR_PR_SKIP_RANKS), and in the last fused iteration 12 B
+    # for every vertex (x written too, also for the vertices without
+    # out-edges), averaged over the profiled launches
     bytes_edgemap = pr_pull_bytes(n, m, 4)
     split = G.tune_get("pr_split_bytes")
     index = np.empty(n + 1, np.uint64)
@@ -179,7 +179,7 @@ def run(args):
     n_linked = int(np.count_nonzero(np.diff(index)))
     del index
     k = args.pr_iters
-    tail_bytes = 12 * ((k - 2) * n_linked + n) if split == 0 and k > 1 else 0
+    tail_bytes = 8 * (k - 2) * n_linked + 12 * n if split == 0 and k > 1 else 0
     bytes_pull = bytes_edgemap + tail_bytes / max(1, k)
     achieved = bytes_pull / (avg_pull_ms * 1e-3) / 1e9
     gteps = {
